@@ -1,0 +1,74 @@
+/*
+ * sicc_decode_ext.h -- entry points of libsicc_amd.so beyond the reference's
+ * cuda_osd ABI (matlab_decoder_c_api.h).  Each one states the reference
+ * behaviour it exposes, or that it has no counterpart in the reference.
+ * Status codes are CUDA_OSD_STATUS_*; messages go to cuda_osd_get_last_error().
+ */
+#ifndef SICC_DECODE_EXT_H
+#define SICC_DECODE_EXT_H
+
+#include "matlab_decoder_c_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Stream BP-OSD decoder on an alist file (LINK:2722-2945 read_alist + build_ldpc
+ * with systematic-front info, as the reference runs with --systematic-front-info). */
+int sicc_create_stream_decoder_alist(const cuda_osd_stream_config_t* config, const char* alist_path,
+                                     unsigned long long* out_handle);
+
+/* Stream BP-OSD decoder on a caller CSR parity-check matrix (rows in the caller's
+ * column order), systematic-front layout built by GF(2) RREF (LINK:2811-2945). */
+int sicc_create_stream_decoder_csr(const cuda_osd_stream_config_t* config, int n, int m,
+                                   const int* row_ptr, const int* col_ind,
+                                   unsigned long long* out_handle);
+
+/* Seeded quasi-cyclic LDPC code: 16 block columns of (n/16)x(n/16) circulant
+ * permutations, mb block rows, column weight col_weight.  No reference
+ * counterpart (the reference only loads alists); it builds the mixed-code
+ * workloads.  The CSR is written to caller buffers when they are non-null:
+ * row_ptr[mb*n/16 + 1], col_ind[col_weight*n]. */
+int sicc_qc_code_csr(int n, int mb, int col_weight, unsigned long long seed, int* out_m,
+                     int* out_nnz, int* row_ptr, int* col_ind);
+
+/* OSD parameters of a stream or OSD-only decoder: order 0..3, flip pool size,
+ * test cap (the MATLAB reference options osdOrder/osdMaxFlipPositions/osdMaxTests,
+ * source/matlab/uhd_cpp/matlab/cpu_bp_osd_decode.m:83-99).  order < 0 disables
+ * OSD.  Defaults: stream decoders order 2 / 32 / 4096 (the MATLAB defaults);
+ * OSD-only decoders order 2 / k / 4096. */
+int sicc_set_osd_params(unsigned long long handle, int order, int max_flip, int max_tests);
+
+/* Code dimensions of a decoder handle. */
+int sicc_get_code_info(unsigned long long handle, int* n, int* k, int* m, int* nnz);
+/* CSR of the decoder's parity-check matrix (row_ptr[m+1], col_ind[nnz]). */
+int sicc_get_code_csr(unsigned long long handle, int* row_ptr, int* col_ind);
+/* Dense generator [k][n] of the decoder's code (row i = encode(e_i)). */
+int sicc_get_generator(unsigned long long handle, unsigned char* out_gen);
+/* Systematic encoder (LINK:2947-2997): info [count][k] -> codewords [count][n]. */
+int sicc_encode(unsigned long long handle, const unsigned char* info, int count, unsigned char* out_cw);
+
+/* Device-resident decode of `codewords` LLR rows already in HBM, on
+ * `hip_stream` (null = the decoder's own stream), asynchronous: BP
+ * (LINK:3013-3094 semantics) then OSD on frames BP does not accept.  Every
+ * output is a device pointer and may be null.  The stream/poll ABI of the
+ * reference hands results back through host memory; this entry point keeps them
+ * on the device for callers that chain further GPU work.  No counterpart in the
+ * reference. */
+int sicc_decode_device(unsigned long long handle, const float* device_llrs, int codewords,
+                       unsigned char* d_info_bits, unsigned char* d_code_bits, float* d_app_llrs,
+                       unsigned char* d_bp_success, unsigned char* d_used_osd, int* d_iterations,
+                       float* d_distances, void* hip_stream);
+
+/* Synchronous host-buffer variant of sicc_decode_device (hard bits, soft LLRs,
+ * per-frame success) for CPU callers; outputs may be null. */
+int sicc_decode_host(unsigned long long handle, const float* llrs, int codewords,
+                     unsigned char* info_bits, unsigned char* code_bits, float* app_llrs,
+                     unsigned char* bp_success, unsigned char* used_osd, int* iterations,
+                     float* distances);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SICC_DECODE_EXT_H */

@@ -1,0 +1,551 @@
+/*
+ * ldpc_oracle.c -- CPU restatement of the reference LDPC code model, normalized
+ * min-sum BP and OSD.  TEST INFRASTRUCTURE ONLY: used by tests/, the smoke test
+ * and the CPU-baseline leg of bench.py as the checker; the product never links it.
+ *
+ * Sources restated (paths relative to the reference repository root):
+ *   LINK = source/cpp_cuda/uhd_cpp/src/uhd_ldpc_ofdm_link.cpp
+ *   MOSD = source/matlab/uhd_cpp/matlab/cpu_bp_osd_decode.m
+ *
+ * Floating point: compiled with -ffp-contract=off so every float operation
+ * rounds exactly as written, in the order written.
+ */
+#include "sicc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* alist reader: LINK:2722-2770 (read_alist).  Rows keep the file's column    */
+/* order; entries beyond the declared weight or equal to 0 are skipped.       */
+/* Two-call use: with row_ptr == NULL only n, m and the edge count are set.   */
+/* ------------------------------------------------------------------------ */
+int oracle_alist_parse(const char* path, int* n_out, int* m_out, int* e_out,
+                       int* row_ptr, int* col_ind, int edge_cap)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) return -1;
+    int n = 0, m = 0, maxc = 0, maxr = 0;
+    if (fscanf(f, "%d %d %d %d", &n, &m, &maxc, &maxr) != 4 || n <= 0 || m <= 0) {
+        fclose(f);
+        return -2;
+    }
+    int* cw = (int*)calloc((size_t)n, sizeof(int));
+    int* rw = (int*)calloc((size_t)m, sizeof(int));
+    for (int i = 0; i < n; ++i) if (fscanf(f, "%d", &cw[i]) != 1) goto bad;
+    for (int i = 0; i < m; ++i) if (fscanf(f, "%d", &rw[i]) != 1) goto bad;
+    for (int c = 0; c < n; ++c)
+        for (int j = 0; j < maxc; ++j) {
+            int r;
+            if (fscanf(f, "%d", &r) != 1) goto bad;
+        }
+    {
+        int e = 0;
+        if (row_ptr) row_ptr[0] = 0;
+        for (int r = 0; r < m; ++r) {
+            for (int j = 0; j < maxr; ++j) {
+                int c = 0;
+                if (fscanf(f, "%d", &c) != 1) goto bad;
+                if (j < rw[r] && c > 0) {
+                    if (col_ind) {
+                        if (e >= edge_cap) goto bad;
+                        col_ind[e] = c - 1;
+                    }
+                    ++e;
+                }
+            }
+            if (row_ptr) row_ptr[r + 1] = e;
+        }
+        *n_out = n;
+        *m_out = m;
+        *e_out = e;
+    }
+    free(cw);
+    free(rw);
+    fclose(f);
+    return 0;
+bad:
+    free(cw);
+    free(rw);
+    fclose(f);
+    return -3;
+}
+
+/* ------------------------------------------------------------------------ */
+/* build_ldpc: LINK:2811-2945.  GF(2) RREF over 64-bit packed rows; with     */
+/* front_info the parity columns (n-m .. n-1) are tried as pivots first.     */
+/* The DVB-S2 accumulator shape (LINK:2834-2866) is detected first.          */
+/* ------------------------------------------------------------------------ */
+static int get_bit(const uint64_t* row, int col) { return (int)((row[col >> 6] >> (col & 63)) & 1u); }
+static void flip_bit(uint64_t* row, int col) { row[col >> 6] ^= (1ull << (col & 63)); }
+
+static int cmp_int(const void* a, const void* b)
+{
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+static int is_accumulator(int n, int m, const int* row_ptr, const int* col_ind, int front_info)
+{
+    if (!front_info || n <= m) return 0;
+    int k = n - m;
+    int buf[64];
+    for (int r = 0; r < m; ++r) {
+        int cnt = 0;
+        for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+            if (col_ind[e] >= k) {
+                if (cnt >= 64) return 0;
+                buf[cnt++] = col_ind[e] - k;
+            }
+        }
+        qsort(buf, (size_t)cnt, sizeof(int), cmp_int);
+        if (r == 0) {
+            if (!(cnt == 1 && buf[0] == 0)) return 0;
+        } else {
+            if (!(cnt == 2 && buf[0] == r - 1 && buf[1] == r)) return 0;
+        }
+    }
+    return 1;
+}
+
+int oracle_build_ldpc(int n, int m, const int* row_ptr, const int* col_ind, int front_info,
+                      int* k_out, int* info_cols, int* pivot_cols, uint64_t* rref_rows,
+                      int* accumulator_out)
+{
+    const int words = (n + 63) / 64;
+    *accumulator_out = 0;
+    if (is_accumulator(n, m, row_ptr, col_ind, front_info)) {
+        *accumulator_out = 1;
+        int k = n - m;
+        for (int i = 0; i < k; ++i) info_cols[i] = i;
+        for (int r = 0; r < m; ++r) pivot_cols[r] = k + r;
+        *k_out = k;
+        return 0;
+    }
+    uint64_t* rows = rref_rows;
+    memset(rows, 0, sizeof(uint64_t) * (size_t)m * (size_t)words);
+    for (int r = 0; r < m; ++r)
+        for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) flip_bit(rows + (size_t)r * words, col_ind[e]);
+
+    int* order = (int*)malloc(sizeof(int) * (size_t)n);
+    int no = 0;
+    if (front_info) {
+        if (n <= m) {
+            free(order);
+            return -2;
+        }
+        for (int c = n - m; c < n; ++c) order[no++] = c;
+        for (int c = 0; c < n - m; ++c) order[no++] = c;
+    } else {
+        for (int c = 0; c < n; ++c) order[no++] = c;
+    }
+    uint64_t* tmp = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)words);
+    int rank = 0;
+    for (int oi = 0; oi < n; ++oi) {
+        if (rank >= m) break;
+        int col = order[oi];
+        int piv = -1;
+        for (int r = rank; r < m; ++r)
+            if (get_bit(rows + (size_t)r * words, col)) {
+                piv = r;
+                break;
+            }
+        if (piv < 0) continue;
+        if (piv != rank) {
+            memcpy(tmp, rows + (size_t)piv * words, sizeof(uint64_t) * words);
+            memcpy(rows + (size_t)piv * words, rows + (size_t)rank * words, sizeof(uint64_t) * words);
+            memcpy(rows + (size_t)rank * words, tmp, sizeof(uint64_t) * words);
+        }
+        for (int r = 0; r < m; ++r)
+            if (r != rank && get_bit(rows + (size_t)r * words, col))
+                for (int w = 0; w < words; ++w) rows[(size_t)r * words + w] ^= rows[(size_t)rank * words + w];
+        pivot_cols[rank] = col;
+        ++rank;
+    }
+    free(tmp);
+    free(order);
+    if (rank != m) return -1;
+    unsigned char* isp = (unsigned char*)calloc((size_t)n, 1);
+    for (int i = 0; i < rank; ++i) isp[pivot_cols[i]] = 1;
+    int k = 0;
+    for (int c = 0; c < n; ++c)
+        if (!isp[c]) info_cols[k++] = c;
+    free(isp);
+    *k_out = k;
+    return 0;
+}
+
+/* ldpc_encode: LINK:2947-2997. */
+void oracle_encode(int n, int m, int k, int accumulator, const int* row_ptr, const int* col_ind,
+                   const int* info_cols, const int* pivot_cols, const uint64_t* rref_rows,
+                   const uint8_t* info, uint8_t* cw)
+{
+    memset(cw, 0, (size_t)n);
+    if (accumulator) {
+        for (int i = 0; i < k; ++i) cw[i] = info[i] & 1u;
+        uint8_t prev = 0;
+        for (int r = 0; r < m; ++r) {
+            uint8_t s = 0;
+            for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+                if (col_ind[e] < k) s ^= cw[col_ind[e]];
+            uint8_t p = (uint8_t)(s ^ (r == 0 ? 0u : prev));
+            cw[k + r] = p;
+            prev = p;
+        }
+        return;
+    }
+    const int words = (n + 63) / 64;
+    uint64_t wbuf[64];
+    uint64_t* w = words <= 64 ? wbuf : (uint64_t*)malloc(sizeof(uint64_t) * (size_t)words);
+    memset(w, 0, sizeof(uint64_t) * (size_t)words);
+    for (int i = 0; i < k; ++i)
+        if (info[i] & 1u) {
+            cw[info_cols[i]] = 1;
+            flip_bit(w, info_cols[i]);
+        }
+    for (int r = 0; r < m; ++r) {
+        int par = 0;
+        for (int x = 0; x < words; ++x) par ^= __builtin_popcountll(rref_rows[(size_t)r * words + x] & w[x]) & 1;
+        if (par) {
+            cw[pivot_cols[r]] = 1;
+            flip_bit(w, pivot_cols[r]);
+        }
+    }
+    if (w != wbuf) free(w);
+}
+
+void oracle_encode_batch(int count, int n, int m, int k, int accumulator, const int* row_ptr,
+                         const int* col_ind, const int* info_cols, const int* pivot_cols,
+                         const uint64_t* rref_rows, const uint8_t* info, uint8_t* cw)
+{
+    for (int i = 0; i < count; ++i)
+        oracle_encode(n, m, k, accumulator, row_ptr, col_ind, info_cols, pivot_cols, rref_rows,
+                      info + (size_t)i * k, cw + (size_t)i * n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Normalized min-sum, flooding schedule: LINK:3013-3094 (ldpc_decode_min_sum)*/
+/* with the MATLAB additions MOSD:115-177: best-syndrome APP tracking and    */
+/* VN-message damping.  Edge order is CSR row order; the VN sum adds the     */
+/* check messages of a column in increasing row order, as LINK's col_edges.  */
+/* offset: magnitude becomes max(mag - offset, 0) before normalization.      */
+/* early_stop = 0 runs all max_iter iterations (a test knob; LINK stops).    */
+/* Returns 1 when the final hard decision satisfies every check.             */
+/* ------------------------------------------------------------------------ */
+static int syndrome_weight(int m, const int* row_ptr, const int* col_ind, const uint8_t* hard)
+{
+    int w = 0;
+    for (int r = 0; r < m; ++r) {
+        int s = 0;
+        for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) s ^= hard[col_ind[e]];
+        w += s;
+    }
+    return w;
+}
+
+int oracle_nms_decode(int n, int m, const int* row_ptr, const int* col_ind, const float* llr,
+                      int max_iter, float alpha, float offset, float damping, int early_stop,
+                      uint8_t* hard_out, float* app_out, float* best_app_out,
+                      int* iters_out, int* best_synw_out)
+{
+    const int E = row_ptr[m];
+    float* q = (float*)malloc(sizeof(float) * (size_t)E);
+    float* r = (float*)calloc((size_t)E, sizeof(float));
+    float* app = (float*)malloc(sizeof(float) * (size_t)n);
+    uint8_t* hard = (uint8_t*)malloc((size_t)n);
+    /* column lists of edge ids, ascending row (= ascending edge id) */
+    int* cdeg = (int*)calloc((size_t)n + 1, sizeof(int));
+    int* cedge = (int*)malloc(sizeof(int) * (size_t)E);
+    for (int e = 0; e < E; ++e) cdeg[col_ind[e] + 1]++;
+    for (int c = 0; c < n; ++c) cdeg[c + 1] += cdeg[c];
+    int* fill = (int*)calloc((size_t)n, sizeof(int));
+    for (int e = 0; e < E; ++e) {
+        int c = col_ind[e];
+        cedge[cdeg[c] + fill[c]++] = e;
+    }
+    free(fill);
+
+    for (int e = 0; e < E; ++e) q[e] = llr[col_ind[e]];
+    for (int c = 0; c < n; ++c) {
+        app[c] = llr[c];
+        hard[c] = llr[c] < 0.0f ? 1 : 0;
+    }
+    int best_w = syndrome_weight(m, row_ptr, col_ind, hard);
+    if (best_app_out) memcpy(best_app_out, llr, sizeof(float) * (size_t)n);
+    int ok = 0, used = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        used = it + 1;
+        for (int row = 0; row < m; ++row) {
+            float min1 = FLT_MAX, min2 = FLT_MAX;
+            int min1_edge = -1, sign_prod = 1;
+            for (int e = row_ptr[row]; e < row_ptr[row + 1]; ++e) {
+                const float v = q[e];
+                if (v < 0.0f) sign_prod = -sign_prod;
+                const float av = fabsf(v);
+                if (av < min1) {
+                    min2 = min1;
+                    min1 = av;
+                    min1_edge = e;
+                } else if (av < min2) {
+                    min2 = av;
+                }
+            }
+            for (int e = row_ptr[row]; e < row_ptr[row + 1]; ++e) {
+                const int s = (q[e] < 0.0f) ? -sign_prod : sign_prod;
+                float mag = (e == min1_edge) ? min2 : min1;
+                if (offset != 0.0f) mag = fmaxf(mag - offset, 0.0f);
+                r[e] = alpha * (float)s * mag;
+            }
+        }
+        for (int c = 0; c < n; ++c) {
+            float v = llr[c];
+            for (int j = cdeg[c]; j < cdeg[c + 1]; ++j) v += r[cedge[j]];
+            app[c] = v;
+            hard[c] = (v < 0.0f) ? 1 : 0;
+        }
+        const int w = syndrome_weight(m, row_ptr, col_ind, hard);
+        if (w < best_w || w == 0) {
+            best_w = w;
+            if (best_app_out) memcpy(best_app_out, app, sizeof(float) * (size_t)n);
+        }
+        if (w == 0) {
+            ok = 1;
+            if (early_stop) break;
+        } else {
+            ok = 0;
+        }
+        for (int c = 0; c < n; ++c)
+            for (int j = cdeg[c]; j < cdeg[c + 1]; ++j) {
+                const int e = cedge[j];
+                const float fresh = app[c] - r[e];
+                q[e] = (damping != 0.0f) ? (1.0f - damping) * fresh + damping * q[e] : fresh;
+            }
+    }
+    if (hard_out) memcpy(hard_out, hard, (size_t)n);
+    if (app_out) memcpy(app_out, app, sizeof(float) * (size_t)n);
+    if (iters_out) *iters_out = used;
+    if (best_synw_out) *best_synw_out = best_w;
+    free(q);
+    free(r);
+    free(app);
+    free(hard);
+    free(cdeg);
+    free(cedge);
+    return ok;
+}
+
+/* ------------------------------------------------------------------------ */
+/* OSD: MOSD:187-245 (osd_decode), MOSD:247-250 (encode_and_metric),         */
+/* MOSD:270-305 (generator_to_systematic).  Dense byte matrices, literal     */
+/* restatement.  MATLAB sort is stable (ties keep the lower index first).    */
+/* Metric: sum of |L| over disagreeing positions, in double, accumulated in  */
+/* ascending systematic position order starting from 0.0.                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double key;
+    int idx;
+} keyed_t;
+
+static int cmp_desc(const void* a, const void* b)
+{
+    const keyed_t* x = (const keyed_t*)a;
+    const keyed_t* y = (const keyed_t*)b;
+    if (x->key > y->key) return -1;
+    if (x->key < y->key) return 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+static int cmp_asc(const void* a, const void* b)
+{
+    const keyed_t* x = (const keyed_t*)a;
+    const keyed_t* y = (const keyed_t*)b;
+    if (x->key < y->key) return -1;
+    if (x->key > y->key) return 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+static double encode_and_metric(int n, int k, const uint8_t* gsys, const uint8_t* msg,
+                                const uint8_t* hard_sys, const double* rel_sys, uint8_t* code)
+{
+    for (int j = 0; j < n; ++j) {
+        uint8_t b = 0;
+        for (int i = 0; i < k; ++i) b ^= (uint8_t)(msg[i] & gsys[(size_t)i * n + j]);
+        code[j] = b;
+    }
+    double metric = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (code[j] != hard_sys[j]) metric += rel_sys[j];
+    return metric;
+}
+
+int oracle_osd_decode(int n, int k, const uint8_t* gen, const float* llr, int order,
+                      int max_flip, int max_tests, uint8_t* code_out, double* metric_out,
+                      int* tests_out)
+{
+    keyed_t* kv = (keyed_t*)malloc(sizeof(keyed_t) * (size_t)n);
+    int* rperm = (int*)malloc(sizeof(int) * (size_t)n);
+    int* sperm = (int*)malloc(sizeof(int) * (size_t)n);
+    uint8_t* g = (uint8_t*)malloc((size_t)k * (size_t)n);
+    uint8_t* hard_sys = (uint8_t*)malloc((size_t)n);
+    double* rel_sys = (double*)malloc(sizeof(double) * (size_t)n);
+    uint8_t* msg = (uint8_t*)malloc((size_t)k);
+    uint8_t* base = (uint8_t*)malloc((size_t)k);
+    uint8_t* cand = (uint8_t*)malloc((size_t)n);
+    uint8_t* best = (uint8_t*)malloc((size_t)n);
+    int status = 0;
+
+    /* [~, reliabilityPerm] = sort(abs(llr), 'descend') -- stable */
+    for (int i = 0; i < n; ++i) {
+        kv[i].key = fabs((double)llr[i]);
+        kv[i].idx = i;
+    }
+    qsort(kv, (size_t)n, sizeof(keyed_t), cmp_desc);
+    for (int i = 0; i < n; ++i) rperm[i] = kv[i].idx;
+
+    /* Grel = G(:, reliabilityPerm) */
+    for (int r = 0; r < k; ++r)
+        for (int j = 0; j < n; ++j) g[(size_t)r * n + j] = gen[(size_t)r * n + rperm[j]] & 1u;
+
+    /* generator_to_systematic: MOSD:270-305 */
+    for (int j = 0; j < n; ++j) sperm[j] = j;
+    for (int t = 0; t < k; ++t) {
+        int prow = -1, pcol = -1;
+        for (int col = t; col < n && prow < 0; ++col)
+            for (int r = t; r < k; ++r)
+                if (g[(size_t)r * n + col]) {
+                    prow = r;
+                    pcol = col;
+                    break;
+                }
+        if (prow < 0) {
+            status = -1; /* rank deficient generator */
+            goto done;
+        }
+        if (prow != t)
+            for (int j = 0; j < n; ++j) {
+                uint8_t x = g[(size_t)t * n + j];
+                g[(size_t)t * n + j] = g[(size_t)prow * n + j];
+                g[(size_t)prow * n + j] = x;
+            }
+        if (pcol != t) {
+            for (int r = 0; r < k; ++r) {
+                uint8_t x = g[(size_t)r * n + t];
+                g[(size_t)r * n + t] = g[(size_t)r * n + pcol];
+                g[(size_t)r * n + pcol] = x;
+            }
+            int x = sperm[t];
+            sperm[t] = sperm[pcol];
+            sperm[pcol] = x;
+        }
+        for (int r = 0; r < k; ++r)
+            if (r != t && g[(size_t)r * n + t])
+                for (int j = 0; j < n; ++j) g[(size_t)r * n + j] ^= g[(size_t)t * n + j];
+    }
+
+    /* hardSys = hardRel(sysPerm), relSys = abs(llrRel(sysPerm)) */
+    for (int j = 0; j < n; ++j) {
+        const int src = rperm[sperm[j]];
+        hard_sys[j] = llr[src] < 0.0f ? 1 : 0;
+        rel_sys[j] = fabs((double)llr[src]);
+    }
+    for (int i = 0; i < k; ++i) base[i] = hard_sys[i];
+    double best_metric = encode_and_metric(n, k, g, base, hard_sys, rel_sys, best);
+    int tests = 1;
+
+    /* flip pool: weakest info positions, stable ascending sort (MOSD:208-209) */
+    for (int i = 0; i < k; ++i) {
+        kv[i].key = rel_sys[i];
+        kv[i].idx = i;
+    }
+    qsort(kv, (size_t)k, sizeof(keyed_t), cmp_asc);
+    const int pool = k < max_flip ? k : max_flip;
+
+    for (int ord = 1; ord <= order; ++ord) {
+        if (pool < ord || tests >= max_tests) break;
+        int c[64];
+        if (ord > 64) break;
+        for (int i = 0; i < ord; ++i) c[i] = i;
+        for (;;) {
+            memcpy(msg, base, (size_t)k);
+            for (int i = 0; i < ord; ++i) msg[kv[c[i]].idx] ^= 1u;
+            const double metric = encode_and_metric(n, k, g, msg, hard_sys, rel_sys, cand);
+            ++tests;
+            if (metric < best_metric) {
+                best_metric = metric;
+                memcpy(best, cand, (size_t)n);
+            }
+            if (tests >= max_tests) break;
+            /* next combination in lexicographic order (nchoosek row order) */
+            int i = ord - 1;
+            while (i >= 0 && c[i] == pool - ord + i) --i;
+            if (i < 0) break;
+            ++c[i];
+            for (int j = i + 1; j < ord; ++j) c[j] = c[j - 1] + 1;
+        }
+    }
+
+    /* codeRel(sysPerm) = bestSys; codeBits(reliabilityPerm) = codeRel */
+    for (int j = 0; j < n; ++j) code_out[rperm[sperm[j]]] = best[j];
+    if (metric_out) *metric_out = best_metric;
+    if (tests_out) *tests_out = tests;
+done:
+    free(kv);
+    free(rperm);
+    free(sperm);
+    free(g);
+    free(hard_sys);
+    free(rel_sys);
+    free(msg);
+    free(base);
+    free(cand);
+    free(best);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BP then OSD on BP failure, per frame: MOSD:55-77.  The OSD input is the   */
+/* best-syndrome APP LLR (MOSD:65, MOSD:158-172).  `threads` > 1 uses OpenMP */
+/* over codewords (the all-cores CPU baseline).                               */
+/* ------------------------------------------------------------------------ */
+int oracle_bposd_batch(int count, int n, int m, int k, const int* row_ptr, const int* col_ind,
+                       const uint8_t* gen, const int* info_cols, const float* llr, int max_iter,
+                       float alpha, float offset, float damping, int osd_order, int max_flip,
+                       int max_tests, int threads, uint8_t* code_out, uint8_t* info_out,
+                       uint8_t* bp_ok_out, uint8_t* used_osd_out, int* iters_out)
+{
+    int failures = 0;
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 16) reduction(+ : failures)
+#endif
+    for (int i = 0; i < count; ++i) {
+        float* best = (float*)malloc(sizeof(float) * (size_t)n);
+        uint8_t* code = code_out + (size_t)i * n;
+        int iters = 0, bw = 0;
+        const int ok = oracle_nms_decode(n, m, row_ptr, col_ind, llr + (size_t)i * n, max_iter, alpha,
+                                         offset, damping, 1, code, NULL, best, &iters, &bw);
+        int used = 0;
+        if (!ok && osd_order >= 0) {
+            used = 1;
+            double met = 0.0;
+            int tests = 0;
+            if (oracle_osd_decode(n, k, gen, best, osd_order, max_flip, max_tests, code, &met, &tests) != 0)
+                failures += 1;
+        }
+        if (info_out)
+            for (int j = 0; j < k; ++j) info_out[(size_t)i * k + j] = code[info_cols[j]];
+        if (bp_ok_out) bp_ok_out[i] = (uint8_t)ok;
+        if (used_osd_out) used_osd_out[i] = (uint8_t)used;
+        if (iters_out) iters_out[i] = iters;
+        free(best);
+    }
+    (void)threads;
+    return failures ? -1 : 0;
+}

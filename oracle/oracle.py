@@ -1,0 +1,144 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke test and bench.py's
+cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+u8p = C.POINTER(C.c_uint8)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int)
+u64p = C.POINTER(C.c_uint64)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run oracle/build.py")
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def alist_parse(path: str):
+    lib = load()
+    n, m, e = C.c_int(), C.c_int(), C.c_int()
+    r = lib.oracle_alist_parse(path.encode(), C.byref(n), C.byref(m), C.byref(e), None, None, 0)
+    if r != 0:
+        raise ValueError(f"alist parse failed ({r})")
+    rp = np.zeros(m.value + 1, np.int32)
+    ci = np.zeros(e.value, np.int32)
+    r = lib.oracle_alist_parse(path.encode(), C.byref(n), C.byref(m), C.byref(e), _p(rp, i32p), _p(ci, i32p),
+                               e.value)
+    if r != 0:
+        raise ValueError(f"alist parse failed ({r})")
+    return n.value, m.value, rp, ci
+
+
+class Code:
+    """Oracle code model: RREF encoder tables (LINK:2811-2997)."""
+
+    def __init__(self, n, m, row_ptr, col_ind, front_info=True):
+        lib = load()
+        self.n, self.m = int(n), int(m)
+        self.rp = np.ascontiguousarray(row_ptr, np.int32)
+        self.ci = np.ascontiguousarray(col_ind, np.int32)
+        self.words = (self.n + 63) // 64
+        self.info_cols = np.zeros(self.n, np.int32)
+        self.pivot_cols = np.zeros(self.m, np.int32)
+        self.rref = np.zeros(self.m * self.words, np.uint64)
+        k, acc = C.c_int(), C.c_int()
+        r = lib.oracle_build_ldpc(self.n, self.m, _p(self.rp, i32p), _p(self.ci, i32p), int(front_info), C.byref(k),
+                                  _p(self.info_cols, i32p), _p(self.pivot_cols, i32p), _p(self.rref, u64p),
+                                  C.byref(acc))
+        if r != 0:
+            raise ValueError(f"build_ldpc failed ({r})")
+        self.k = k.value
+        self.acc = acc.value
+        self.info_cols = self.info_cols[: self.k].copy()
+
+    def encode(self, info: np.ndarray) -> np.ndarray:
+        info = np.ascontiguousarray(info, np.uint8).reshape(-1, self.k)
+        out = np.zeros((info.shape[0], self.n), np.uint8)
+        load().oracle_encode_batch(info.shape[0], self.n, self.m, self.k, self.acc, _p(self.rp, i32p),
+                                   _p(self.ci, i32p), _p(self.info_cols, i32p), _p(self.pivot_cols, i32p),
+                                   _p(self.rref, u64p), _p(info, u8p), _p(out, u8p))
+        return out
+
+    def generator(self) -> np.ndarray:
+        return self.encode(np.eye(self.k, dtype=np.uint8))
+
+    def dense_h(self) -> np.ndarray:
+        h = np.zeros((self.m, self.n), np.uint8)
+        for r in range(self.m):
+            h[r, self.ci[self.rp[r]:self.rp[r + 1]]] = 1
+        return h
+
+    def nms(self, llr, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, early_stop=True):
+        llr = np.ascontiguousarray(llr, np.float32)
+        hard = np.zeros(self.n, np.uint8)
+        app = np.zeros(self.n, np.float32)
+        best = np.zeros(self.n, np.float32)
+        it, bw = C.c_int(), C.c_int()
+        ok = load().oracle_nms_decode(self.n, self.m, _p(self.rp, i32p), _p(self.ci, i32p), _p(llr, f32p), max_iter,
+                                      C.c_float(alpha), C.c_float(offset), C.c_float(damping), int(early_stop),
+                                      _p(hard, u8p), _p(app, f32p), _p(best, f32p), C.byref(it), C.byref(bw))
+        return {"ok": bool(ok), "hard": hard, "app": app, "best_app": best, "iterations": it.value,
+                "best_synw": bw.value}
+
+    def bposd(self, llr, gen=None, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, order=1, max_flip=64,
+              max_tests=4096, threads=1):
+        llr = np.ascontiguousarray(llr, np.float32).reshape(-1, self.n)
+        cnt = llr.shape[0]
+        gen = np.ascontiguousarray(self.generator() if gen is None else gen, np.uint8)
+        code = np.zeros((cnt, self.n), np.uint8)
+        info = np.zeros((cnt, self.k), np.uint8)
+        ok = np.zeros(cnt, np.uint8)
+        used = np.zeros(cnt, np.uint8)
+        iters = np.zeros(cnt, np.int32)
+        r = load().oracle_bposd_batch(cnt, self.n, self.m, self.k, _p(self.rp, i32p), _p(self.ci, i32p),
+                                      _p(gen, u8p), _p(self.info_cols, i32p), _p(llr, f32p), max_iter,
+                                      C.c_float(alpha), C.c_float(offset), C.c_float(damping), order, max_flip,
+                                      max_tests, threads, _p(code, u8p), _p(info, u8p), _p(ok, u8p), _p(used, u8p),
+                                      _p(iters, i32p))
+        if r != 0:
+            raise RuntimeError("oracle OSD failed (rank deficient generator)")
+        return {"code": code, "info": info, "bp_ok": ok, "used_osd": used, "iterations": iters}
+
+
+def osd(gen: np.ndarray, llr: np.ndarray, order: int, max_flip: int = 64, max_tests: int = 4096):
+    gen = np.ascontiguousarray(gen, np.uint8)
+    k, n = gen.shape
+    llr = np.ascontiguousarray(llr, np.float32)
+    code = np.zeros(n, np.uint8)
+    met = C.c_double()
+    tests = C.c_int()
+    r = load().oracle_osd_decode(n, k, _p(gen, u8p), _p(llr, f32p), order, max_flip, max_tests, _p(code, u8p),
+                                 C.byref(met), C.byref(tests))
+    if r != 0:
+        raise RuntimeError("rank deficient generator")
+    return code, met.value, tests.value
+
+
+def awgn_llr(codewords: np.ndarray, ebn0_db: float, rate: float, rng: np.random.Generator) -> np.ndarray:
+    """BPSK (0 -> +1) over AWGN, LLR = 2y/sigma^2 with sigma^2 = 1/(2 R Eb/N0)
+    (source/cpp_cuda/uhd_cpp/tests/gpu_bp_osd_awgn_bench.cpp:159-172)."""
+    sigma2 = 1.0 / (2.0 * rate * 10.0 ** (ebn0_db / 10.0))
+    x = 1.0 - 2.0 * codewords.astype(np.float64)
+    y = x + np.sqrt(sigma2) * rng.standard_normal(codewords.shape)
+    return (2.0 / sigma2 * y).astype(np.float32)

@@ -1,0 +1,82 @@
+/*
+ * sicc_oracle.h -- CPU restatement of the reference receive-and-decode path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the parity tests,
+ * the smoke test and the CPU-baseline leg of bench.py compare the HIP product
+ * against.  Nothing in the product (the sicc-baseband-proto_amd package or its
+ * shared library) may link, load or call it.
+ *
+ * Every function names the reference file:line it restates.  Reference paths
+ * are relative to the reference repository root:
+ *   LINK  = source/cpp_cuda/uhd_cpp/src/uhd_ldpc_ofdm_link.cpp
+ *   MOSD  = source/matlab/uhd_cpp/matlab/cpu_bp_osd_decode.m
+ *   PIPE  = source/cpp_cuda/uhd_cpp/src/gpu_ofdm_pipeline.cu
+ *
+ * Pinning status of each function (how it is checked independently of the
+ * product, see tests/test_oracle_*.py):
+ *   oracle_alist_parse      pinned: CCSDS header/degree counts, H*G^T = 0.
+ *   oracle_build_ldpc       pinned: H*G^T = 0, rank, front-info layout.
+ *   oracle_encode           pinned: syndrome of every output is zero, linearity.
+ *   oracle_nms_decode       pinned: single-check closed form, tree code max-log
+ *                           MAP by brute force, 1-iteration decode of clean codewords.
+ *   oracle_osd_decode       pinned: output in code, order-k OSD == brute-force ML,
+ *                           argmin over explicit TEP set, basis independence, ties.
+ *   oracle_ofdm_*           pinned: numpy FFT, closed-form QPSK LLR, brute force
+ *                           sliding metric, noise-free known-answer loopback.
+ */
+#ifndef SICC_ORACLE_H
+#define SICC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- code model (LINK:2722-3160) ---- */
+int oracle_alist_parse(const char* path, int* n_out, int* m_out, int* e_out,
+                       int* row_ptr, int* col_ind, int edge_cap);
+int oracle_build_ldpc(int n, int m, const int* row_ptr, const int* col_ind, int front_info,
+                      int* k_out, int* info_cols, int* pivot_cols, uint64_t* rref_rows,
+                      int* accumulator_out);
+void oracle_encode(int n, int m, int k, int accumulator, const int* row_ptr, const int* col_ind,
+                   const int* info_cols, const int* pivot_cols, const uint64_t* rref_rows,
+                   const uint8_t* info, uint8_t* cw);
+void oracle_encode_batch(int count, int n, int m, int k, int accumulator, const int* row_ptr,
+                         const int* col_ind, const int* info_cols, const int* pivot_cols,
+                         const uint64_t* rref_rows, const uint8_t* info, uint8_t* cw);
+
+/* ---- normalized min-sum BP (LINK:3013-3094, MOSD:101-185) ---- */
+int oracle_nms_decode(int n, int m, const int* row_ptr, const int* col_ind, const float* llr,
+                      int max_iter, float alpha, float offset, float damping, int early_stop,
+                      uint8_t* hard_out, float* app_out, float* best_app_out,
+                      int* iters_out, int* best_synw_out);
+
+/* ---- OSD (MOSD:187-305) ---- */
+int oracle_osd_decode(int n, int k, const uint8_t* gen, const float* llr, int order,
+                      int max_flip, int max_tests, uint8_t* code_out, double* metric_out,
+                      int* tests_out);
+
+/* ---- BP followed by OSD on failure, over a batch (MOSD:55-77) ---- */
+int oracle_bposd_batch(int count, int n, int m, int k, const int* row_ptr, const int* col_ind,
+                       const uint8_t* gen, const int* info_cols, const float* llr, int max_iter,
+                       float alpha, float offset, float damping, int osd_order, int max_flip,
+                       int max_tests, int threads, uint8_t* code_out, uint8_t* info_out,
+                       uint8_t* bp_ok_out, uint8_t* used_osd_out, int* iters_out);
+
+/* ---- OFDM receive (LINK:3788-3860, LINK:5123-5212, LINK:5331-5521) ---- */
+void oracle_fft(double* re, double* im, int n, int inverse);
+int oracle_find_frame(const float* iq, int count, int half_len, const float* preamble,
+                      int pre_len, int refine_radius, double threshold, double fs,
+                      int* start_out, double* peak_out, double* cfo_out, double* coarse_out);
+void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int num_symbols,
+                        int active_sc, const int* used_shifted, int n_pilot, const int* pilot_sym,
+                        const float* pilot_grid, int n_data, const int* data_sym,
+                        const int* interp_p0, const int* interp_p1, const float* interp_alpha,
+                        int bits_per_symbol, double cfo_hz, double fs, double noise_var,
+                        long phase_origin, float* llr_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,143 @@
+// decoder_engine.cpp -- device tables and BP -> OSD orchestration for one code.
+#include "decoder_engine.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+
+namespace sicc {
+
+Engine::Engine(LdpcCode code) : code_(std::move(code))
+{
+    const int n = code_.n, m = code_.m, E = code_.edges();
+    int max_dc = 0;
+    for (int r = 0; r < m; ++r) max_dc = std::max(max_dc, code_.row_ptr[r + 1] - code_.row_ptr[r]);
+    if (n > 0x7fff || E > 0xffff) throw std::invalid_argument("code too large for the short-code BP tables");
+
+    // Column-major slots: a column's edges in increasing row order.
+    std::vector<int> col_cnt(n + 1, 0);
+    for (int e = 0; e < E; ++e) col_cnt[code_.col_ind[e] + 1]++;
+    for (int v = 0; v < n; ++v) col_cnt[v + 1] += col_cnt[v];
+    std::vector<int> fill(n, 0), slot(E);
+    for (int r = 0; r < m; ++r)
+        for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e) {
+            const int v = code_.col_ind[e];
+            slot[e] = col_cnt[v] + fill[v]++;
+        }
+    int max_dv = 0;
+    std::vector<int> col_meta(n);
+    for (int v = 0; v < n; ++v) {
+        const int dv = col_cnt[v + 1] - col_cnt[v];
+        max_dv = std::max(max_dv, dv);
+        if (dv > 255) throw std::invalid_argument("variable degree above 255");
+        col_meta[v] = (col_cnt[v] << 8) | dv;
+    }
+    std::vector<int> row_meta(static_cast<size_t>(m) * std::max(max_dc, 1), -1);
+    for (int r = 0; r < m; ++r) {
+        int j = 0;
+        for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e, ++j)
+            row_meta[static_cast<size_t>(r) * max_dc + j] = (code_.col_ind[e] << 16) | slot[e];
+    }
+    row_meta_.alloc(row_meta.size());
+    col_meta_.alloc(n);
+    info_cols_.alloc(std::max(code_.k, 1));
+    SICC_HIP(hipMemcpy(row_meta_.get(), row_meta.data(), row_meta.size() * sizeof(int), hipMemcpyHostToDevice));
+    SICC_HIP(hipMemcpy(col_meta_.get(), col_meta.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    SICC_HIP(hipMemcpy(info_cols_.get(), code_.info_cols.data(), code_.k * sizeof(int), hipMemcpyHostToDevice));
+
+    bpdev_.n = n;
+    bpdev_.m = m;
+    bpdev_.k = code_.k;
+    bpdev_.E = E;
+    bpdev_.max_dc = max_dc;
+    bpdev_.row_meta = row_meta_.get();
+    bpdev_.col_meta = col_meta_.get();
+    bpdev_.info_cols = info_cols_.get();
+    if (!bp_shape_supported(bpdev_)) throw std::invalid_argument("code shape not supported by the BP kernel");
+
+    osd_ok_ = osd_shape_supported(n, code_.k);
+    if (osd_ok_) {
+        const auto g = code_.dense_generator();
+        std::vector<unsigned long long> cols(n, 0);
+        for (int r = 0; r < code_.k; ++r)
+            for (int c = 0; c < n; ++c)
+                if (g[static_cast<size_t>(r) * n + c]) cols[c] |= 1ull << r;
+        gen_cols_.alloc(n);
+        SICC_HIP(hipMemcpy(gen_cols_.get(), cols.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        osddev_.n = n;
+        osddev_.k = code_.k;
+        osddev_.gen_cols = gen_cols_.get();
+        osddev_.info_cols = info_cols_.get();
+    }
+    osd_count_.alloc(1);
+
+    int dev = 0, cus = 256;
+    SICC_HIP(hipGetDevice(&dev));
+    SICC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    max_waves_ = std::max(cus, 1) * 32;
+}
+
+void Engine::ensure_scratch(int num_cw)
+{
+    if (num_cw <= scratch_cw_) return;
+    // Grow geometrically so a sweep of batch sizes does not reallocate each call.
+    const int cap = std::max(num_cw, scratch_cw_ * 2);
+    osd_llr_.alloc(static_cast<size_t>(cap) * code_.n);
+    osd_index_.alloc(cap);
+    scratch_cw_ = cap;
+}
+
+void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                    const DecodeOut& out, hipStream_t s)
+{
+    if (num_cw <= 0) return;
+    if (!d_llr) throw std::invalid_argument("null LLR pointer");
+    const bool use_osd = osd_order >= 0;
+    if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 64, n <= 256)");
+    BpIo io;
+    io.llr = d_llr;
+    io.num_cw = num_cw;
+    io.code_bits = out.code;
+    io.info_bits = out.info;
+    io.app = out.app;
+    io.success = out.bp_ok;
+    io.iterations = out.iterations;
+    io.used_osd = out.used_osd;
+    io.distance = out.distance;
+    if (use_osd) {
+        ensure_scratch(num_cw);
+        launch_zero_i32(osd_count_.get(), 1, s);
+        io.osd_llr = osd_llr_.get();
+        io.osd_index = osd_index_.get();
+        io.osd_count = osd_count_.get();
+    }
+    launch_bp_flood(bpdev_, bp, io, max_waves_, s);
+    if (use_osd) {
+        OsdParams op = osd;
+        op.order = osd_order;
+        OsdIo oio;
+        oio.llr = osd_llr_.get();
+        oio.index = osd_index_.get();
+        oio.count = osd_count_.get();
+        oio.code_bits = out.code;
+        oio.info_bits = out.info;
+        oio.distance = out.distance;
+        launch_osd(osddev_, op, oio, num_cw, s);
+    }
+}
+
+void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s)
+{
+    if (num_cw <= 0) return;
+    if (!osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 64, n <= 256)");
+    OsdIo oio;
+    oio.llr = d_llr;
+    oio.host_count = num_cw;
+    oio.code_bits = out.code;
+    oio.info_bits = out.info;
+    oio.distance = out.distance;
+    oio.used_osd = out.used_osd;
+    launch_osd(osddev_, osd, oio, num_cw, s);
+}
+
+}  // namespace sicc

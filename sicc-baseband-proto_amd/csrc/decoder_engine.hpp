@@ -1,0 +1,56 @@
+// decoder_engine.hpp -- one LDPC code resident on the GPU: graph tables for BP,
+// the systematic generator for OSD, and the device-side BP -> OSD hand-off.
+// Everything here is stream-ordered and asynchronous; nothing synchronizes
+// the device.
+#pragma once
+
+#include "code_model.hpp"
+#include "kernels.hpp"
+#include "sicc_common.hpp"
+
+namespace sicc {
+
+struct DecodeOut {
+    uint8_t* info = nullptr;      // [cw][k]
+    uint8_t* code = nullptr;      // [cw][n]
+    float* app = nullptr;         // [cw][n] BP APP LLR (final iteration)
+    uint8_t* bp_ok = nullptr;     // [cw]
+    uint8_t* used_osd = nullptr;  // [cw]
+    int* iterations = nullptr;    // [cw]
+    float* distance = nullptr;    // [cw] OSD metric, 0 for BP-accepted frames
+};
+
+class Engine {
+public:
+    explicit Engine(LdpcCode code);
+    const LdpcCode& code() const { return code_; }
+    bool osd_supported() const { return osd_ok_; }
+
+    // BP over num_cw device-resident codewords, then OSD (order >= 0) on the
+    // frames BP did not accept.  osd_order < 0 disables OSD.
+    void decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                const DecodeOut& out, hipStream_t s);
+    // OSD on every codeword (no BP).
+    void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
+
+    int max_waves() const { return max_waves_; }
+
+private:
+    void ensure_scratch(int num_cw);
+
+    LdpcCode code_;
+    bool osd_ok_ = false;
+    int max_waves_ = 8192;
+    DevBuf<int> row_meta_;
+    DevBuf<int> col_meta_;
+    DevBuf<int> info_cols_;
+    DevBuf<unsigned long long> gen_cols_;
+    DevBuf<float> osd_llr_;
+    DevBuf<int> osd_index_;
+    DevBuf<int> osd_count_;
+    int scratch_cw_ = 0;
+    BpCodeDev bpdev_;
+    OsdCodeDev osddev_;
+};
+
+}  // namespace sicc

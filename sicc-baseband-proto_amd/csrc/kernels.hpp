@@ -1,0 +1,90 @@
+// kernels.hpp -- host-visible launchers for the gfx950 decode kernels, and the
+// device-side descriptors they take.  All pointers are device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sicc {
+
+// Parity-check graph of one code, laid out for the BP kernels.
+//   row_meta[r * max_dc + j] = (col << 16) | slot, or -1 past the row degree;
+//     j follows the row's CSR order, slot is the edge's position in the
+//     column-major message array.
+//   col_meta[v] = (first_slot << 8) | degree; a column's slots hold its edges
+//     in increasing row order.
+struct BpCodeDev {
+    int n = 0;
+    int m = 0;
+    int k = 0;
+    int E = 0;
+    int max_dc = 0;
+    const int* row_meta = nullptr;
+    const int* col_meta = nullptr;
+    const int* info_cols = nullptr;  // [k]
+};
+
+struct BpParams {
+    int max_iter = 50;
+    float alpha = 0.75f;     // normalization
+    float offset = 0.0f;     // magnitude offset, mag = max(mag - offset, 0)
+    float damping = 0.0f;    // VN message damping (MATLAB reference option)
+    float min_abs_accept = 0.0f;  // a converged frame needs min|APP| >= this
+};
+
+// Per-launch inputs/outputs of the BP kernel.  Null outputs are skipped.
+struct BpIo {
+    const float* llr = nullptr;   // [num_cw][n]
+    int num_cw = 0;
+    uint8_t* code_bits = nullptr; // [num_cw][n]
+    uint8_t* info_bits = nullptr; // [num_cw][k]
+    float* app = nullptr;         // [num_cw][n] final APP LLR
+    uint8_t* success = nullptr;   // [num_cw] BP syndrome satisfied (and accepted)
+    int* iterations = nullptr;    // [num_cw]
+    uint8_t* used_osd = nullptr;  // [num_cw] 1 when handed to OSD
+    float* distance = nullptr;    // [num_cw] 0 for accepted frames
+    // Hand-off of rejected frames to OSD: best-syndrome APP and frame index.
+    float* osd_llr = nullptr;     // [num_cw][n]
+    int* osd_index = nullptr;     // [num_cw]
+    int* osd_count = nullptr;     // one counter, zeroed before the launch
+};
+
+// OSD inputs: the systematic generator as column words (bit r of gen_cols[c]
+// is G[r][c], k <= 64) and the frames BP handed off.
+struct OsdCodeDev {
+    int n = 0;
+    int k = 0;
+    const unsigned long long* gen_cols = nullptr;  // [n]
+    const int* info_cols = nullptr;                // [k]
+};
+
+struct OsdParams {
+    int order = 1;
+    int max_flip = 64;
+    int max_tests = 4096;
+};
+
+struct OsdIo {
+    const float* llr = nullptr;     // [count][n]
+    const int* index = nullptr;     // [count] frame index, or null = identity
+    const int* count = nullptr;     // device counter, or null = host_count
+    int host_count = 0;
+    uint8_t* code_bits = nullptr;   // [frames][n]
+    uint8_t* info_bits = nullptr;   // [frames][k]
+    float* distance = nullptr;      // [frames] best correlation metric
+    uint8_t* used_osd = nullptr;    // [frames] set to 1 for processed frames
+    int* status = nullptr;          // [frames] 0 ok, 1 rank-deficient generator
+};
+
+// Kernel launchers.  They validate shapes on the host and throw
+// std::invalid_argument when a code does not fit an instantiated variant.
+void launch_zero_i32(int* p, int count, hipStream_t s);
+bool bp_shape_supported(const BpCodeDev& c);
+void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves,
+                     hipStream_t s);
+bool osd_shape_supported(int n, int k);
+void launch_osd(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames,
+                hipStream_t s);
+
+}  // namespace sicc

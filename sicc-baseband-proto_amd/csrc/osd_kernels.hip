@@ -1,0 +1,368 @@
+// osd_kernels.hip -- bit-exact ordered-statistics decoding for gfx950.
+//
+// Semantics follow the MATLAB reference osd_decode / generator_to_systematic
+// (source/matlab/uhd_cpp/matlab/cpu_bp_osd_decode.m:187-305):
+//   1. stable sort of |L| descending, columns of G permuted accordingly;
+//   2. Gaussian elimination to [I | P] where step t takes the first column
+//      c >= t with a one in rows >= t, the first such row, swaps rows and
+//      columns, then clears column t in every other row;
+//   3. base message = hard decisions of the k most reliable independent
+//      positions; test error patterns (TEPs) of order <= p over the
+//      `max_flip` weakest of those positions (stable ascending sort), in
+//      nchoosek order, capped at `max_tests`;
+//   4. metric = sum of |L| over positions where the re-encoded word differs
+//      from the hard decision, accumulated in double in ascending systematic
+//      position order; strict-less keeps the first best TEP.
+//
+// Mapping: one wavefront per frame.  The generator is held column-major: lane l
+// owns columns l, l+64, ... as 64-bit words over the k <= 64 rows, so the
+// pivot search is one ballot per column slot and a column swap is a lane
+// exchange.  After elimination the parity part is transposed into LDS rows
+// R[i] (ballots), and each lane evaluates whole TEPs: the disagreement
+// vector of a TEP is D0 ^ R[f1] ^ R[f2] ..., summed in ascending position
+// order so every lane reproduces the sequential CPU sum exactly.  The winner
+// is the lexicographic minimum of (metric, test index) over the wave.
+#include "kernels.hpp"
+
+#include <stdexcept>
+#include <string>
+
+namespace sicc {
+
+namespace {
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ long long nck(int n, int r)
+{
+    if (r < 0 || n < r) return 0;
+    if (r == 0) return 1;
+    if (r == 1) return n;
+    if (r == 2) return static_cast<long long>(n) * (n - 1) / 2;
+    return static_cast<long long>(n) * (n - 1) * (n - 2) / 6;
+}
+
+// Unrank test index t into flip-pool indices (lexicographic combinations,
+// order blocks 0, 1, 2, 3 in sequence).  Returns the order.
+__device__ __forceinline__ int unrank_tep(long long t, int pool, int order, int* idx)
+{
+    int o = 0;
+    for (; o <= order; ++o) {
+        const long long c = nck(pool, o);
+        if (t < c) break;
+        t -= c;
+    }
+    int prev = -1;
+    for (int pos = 0; pos < o; ++pos) {
+        int a = prev + 1;
+        for (;; ++a) {
+            const long long cnt = nck(pool - a - 1, o - pos - 1);
+            if (t < cnt) break;
+            t -= cnt;
+        }
+        idx[pos] = a;
+        prev = a;
+    }
+    return o;
+}
+
+constexpr int kMaxOrder = 3;
+constexpr int kOsdWavesPerBlock = 4;
+
+template <int CPL>
+struct OsdLds {
+    float llr[64 * CPL];
+    double rel[64 * CPL];
+    unsigned long long rows[64 * CPL];  // R[i][s], i < k
+    int rperm[64 * CPL];
+    int pool[64];
+    unsigned char code[64 * CPL];
+};
+
+template <int CPL>
+__global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, OsdIo io)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    OsdLds<CPL>& sh = reinterpret_cast<OsdLds<CPL>*>(lds_raw)[wv];
+    const int n = code.n;
+    const int k = code.k;
+    const int total = io.count ? *io.count : io.host_count;
+    const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    const int pool = k < p.max_flip ? k : p.max_flip;
+    long long tests_total = 0;
+    for (int o = 0; o <= p.order; ++o) tests_total += nck(pool, o);
+    if (tests_total > p.max_tests) tests_total = p.max_tests;
+
+    for (int f = gw; f < total; f += nw) {
+        const int frame = io.index ? io.index[f] : f;
+        const float* L = io.llr + static_cast<size_t>(f) * n;
+        // 1. load and rank by |L| descending, stable
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+            sh.llr[c] = c < n ? L[c] : 0.0f;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+            if (c < n) {
+                const float a = fabsf(sh.llr[c]);
+                int rank = 0;
+                for (int j = 0; j < n; ++j) {
+                    const float b = fabsf(sh.llr[j]);
+                    rank += (b > a || (b == a && j < c)) ? 1 : 0;
+                }
+                sh.rperm[rank] = c;
+            }
+        }
+        wave_lds_sync();
+        // 2. column words of the permuted generator
+        unsigned long long col[CPL];
+        int sp[CPL];
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+            col[s] = c < n ? code.gen_cols[sh.rperm[c]] : 0ull;
+            sp[s] = c;
+        }
+        int status = 0;
+        for (int t = 0; t < k; ++t) {
+            int pc = -1;
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) {
+                const int c = lane + 64 * s;
+                const bool cand = c >= t && c < n && (col[s] >> t) != 0ull;
+                const unsigned long long b = __ballot(cand);
+                if (pc < 0 && b != 0ull) pc = 64 * s + __ffsll(static_cast<long long>(b)) - 1;
+            }
+            if (pc < 0) {
+                status = 1;
+                break;
+            }
+            // pivot column word (uniform)
+            unsigned long long wpc = 0;
+#pragma unroll
+            for (int s = 0; s < CPL; ++s)
+                if (s == (pc >> 6)) wpc = __shfl(col[s], pc & 63);
+            const int prow = t + __ffsll(static_cast<long long>(wpc >> t)) - 1;
+            if (prow != t) {
+                const unsigned long long mt = 1ull << t, mp = 1ull << prow;
+#pragma unroll
+                for (int s = 0; s < CPL; ++s) {
+                    const bool b1 = (col[s] & mt) != 0ull, b2 = (col[s] & mp) != 0ull;
+                    if (b1 != b2) col[s] ^= (mt | mp);
+                }
+            }
+            if (pc != t) {
+                unsigned long long wt = 0, wp = 0;
+                int st = 0, spc = 0;
+#pragma unroll
+                for (int s = 0; s < CPL; ++s) {
+                    const unsigned long long a = __shfl(col[s], t & 63);
+                    const unsigned long long b = __shfl(col[s], pc & 63);
+                    const int sa = __shfl(sp[s], t & 63);
+                    const int sb = __shfl(sp[s], pc & 63);
+                    if (s == (t >> 6)) {
+                        wt = a;
+                        st = sa;
+                    }
+                    if (s == (pc >> 6)) {
+                        wp = b;
+                        spc = sb;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < CPL; ++s) {
+                    const int c = lane + 64 * s;
+                    if (c == t) {
+                        col[s] = wp;
+                        sp[s] = spc;
+                    } else if (c == pc) {
+                        col[s] = wt;
+                        sp[s] = st;
+                    }
+                }
+            }
+            unsigned long long piv = 0;
+#pragma unroll
+            for (int s = 0; s < CPL; ++s)
+                if (s == (t >> 6)) piv = __shfl(col[s], t & 63);
+            const unsigned long long clr = piv & ~(1ull << t);
+#pragma unroll
+            for (int s = 0; s < CPL; ++s)
+                if ((col[s] >> t) & 1ull) col[s] ^= clr;
+        }
+        if (status) {
+            if (lane == 0 && io.status) io.status[frame] = 1;
+            continue;
+        }
+        // 3. systematic hard decisions and reliabilities
+        bool hs[CPL];
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+            if (c < n) {
+                const float x = sh.llr[sh.rperm[sp[s]]];
+                hs[s] = x < 0.0f;
+                sh.rel[c] = static_cast<double>(fabsf(x));
+            } else {
+                hs[s] = false;
+            }
+        }
+        // base message (k <= 64: all in slot 0)
+        const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
+        const unsigned long long base = __ballot(hs[0]) & kmask;
+        unsigned long long d0[CPL];
+#pragma unroll
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+            const bool par = c >= k && c < n;
+            const bool pbit = (__popcll(base & col[s]) & 1) != 0;
+            d0[s] = __ballot(par && (pbit != hs[s]));
+        }
+        for (int i = 0; i < k; ++i) {
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) {
+                const int c = lane + 64 * s;
+                const unsigned long long b = __ballot(c >= k && c < n && ((col[s] >> i) & 1ull));
+                if (lane == 0) sh.rows[i * CPL + s] = b;
+            }
+        }
+        wave_lds_sync();
+        // weak-order flip pool: stable ascending sort of rel[0..k)
+        for (int c = lane; c < k; c += 64) {
+            const double a = sh.rel[c];
+            int rank = 0;
+            for (int j = 0; j < k; ++j) {
+                const double b = sh.rel[j];
+                rank += (b < a || (b == a && j < c)) ? 1 : 0;
+            }
+            if (rank < pool) sh.pool[rank] = c;
+        }
+        wave_lds_sync();
+        // 4. evaluate TEPs, one per lane at a time
+        double best_m = 0.0;
+        long long best_t = -1;
+        for (long long t = lane; t < tests_total; t += 64) {
+            int ix[kMaxOrder];
+            const int o = unrank_tep(t, pool, p.order, ix);
+            int pos[kMaxOrder];
+            for (int i = 0; i < o; ++i) pos[i] = sh.pool[ix[i]];
+            // ascending position order for the sum
+            for (int i = 1; i < o; ++i)
+                for (int j = i; j > 0 && pos[j - 1] > pos[j]; --j) {
+                    const int x = pos[j];
+                    pos[j] = pos[j - 1];
+                    pos[j - 1] = x;
+                }
+            double metric = 0.0;
+            unsigned long long d[CPL];
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) d[s] = d0[s];
+            for (int i = 0; i < o; ++i) {
+                metric += sh.rel[pos[i]];
+#pragma unroll
+                for (int s = 0; s < CPL; ++s) d[s] ^= sh.rows[pos[i] * CPL + s];
+            }
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) {
+                unsigned long long w = d[s];
+                while (w) {
+                    const int b = __ffsll(static_cast<long long>(w)) - 1;
+                    metric += sh.rel[64 * s + b];
+                    w &= w - 1ull;
+                }
+            }
+            if (best_t < 0 || metric < best_m) {
+                best_m = metric;
+                best_t = t;
+            }
+        }
+        // lexicographic (metric, index) minimum over the wave
+        for (int off = 32; off > 0; off >>= 1) {
+            const double om = __shfl_xor(best_m, off);
+            const long long ot = __shfl_xor(best_t, off);
+            const bool take = ot >= 0 && (best_t < 0 || om < best_m || (om == best_m && ot < best_t));
+            if (take) {
+                best_m = om;
+                best_t = ot;
+            }
+        }
+        // 5. rebuild the winner and un-permute
+        {
+            int ix[kMaxOrder];
+            const int o = unrank_tep(best_t, pool, p.order, ix);
+            unsigned long long fm = 0;
+            unsigned long long d[CPL];
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) d[s] = d0[s];
+            for (int i = 0; i < o; ++i) {
+                const int ps = sh.pool[ix[i]];
+                fm |= 1ull << ps;
+#pragma unroll
+                for (int s = 0; s < CPL; ++s) d[s] ^= sh.rows[ps * CPL + s];
+            }
+#pragma unroll
+            for (int s = 0; s < CPL; ++s) {
+                const int c = lane + 64 * s;
+                if (c < n) {
+                    unsigned char bit;
+                    if (c < k)
+                        bit = static_cast<unsigned char>(((base ^ fm) >> c) & 1ull);
+                    else
+                        bit = static_cast<unsigned char>((hs[s] ? 1 : 0) ^ ((d[s] >> lane) & 1ull));
+                    sh.code[sh.rperm[sp[s]]] = bit;
+                }
+            }
+        }
+        wave_lds_sync();
+        if (io.code_bits)
+            for (int c = lane; c < n; c += 64) io.code_bits[static_cast<size_t>(frame) * n + c] = sh.code[c];
+        if (io.info_bits)
+            for (int i = lane; i < k; i += 64)
+                io.info_bits[static_cast<size_t>(frame) * k + i] = sh.code[code.info_cols[i]];
+        if (lane == 0) {
+            if (io.distance) io.distance[frame] = static_cast<float>(best_m);
+            if (io.used_osd) io.used_osd[frame] = 1;
+            if (io.status) io.status[frame] = 0;
+        }
+        wave_lds_sync();
+    }
+}
+
+template <int CPL>
+void launch_osd_variant(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames, hipStream_t s)
+{
+    int waves = max_frames < 2048 ? max_frames : 2048;
+    if (waves < 1) waves = 1;
+    const int blocks = (waves + kOsdWavesPerBlock - 1) / kOsdWavesPerBlock;
+    const size_t lds = sizeof(OsdLds<CPL>) * kOsdWavesPerBlock;
+    hipLaunchKernelGGL((osd_kernel<CPL>), dim3(blocks), dim3(64 * kOsdWavesPerBlock), lds, s, code, p, io);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("osd_kernel launch: ") + hipGetErrorString(e));
+}
+
+}  // namespace
+
+bool osd_shape_supported(int n, int k) { return n > 0 && k > 0 && k <= 64 && k < n && n <= 256; }
+
+void launch_osd(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames, hipStream_t s)
+{
+    if (!osd_shape_supported(code.n, code.k)) throw std::invalid_argument("OSD supports k <= 64, n <= 256");
+    if (p.order < 0 || p.order > kMaxOrder) throw std::invalid_argument("OSD order must be 0..3");
+    if (p.max_flip < 1 || p.max_tests < 1) throw std::invalid_argument("OSD max_flip and max_tests must be >= 1");
+    if (max_frames <= 0) return;
+    if (code.n <= 128) return launch_osd_variant<2>(code, p, io, max_frames, s);
+    return launch_osd_variant<4>(code, p, io, max_frames, s);
+}
+
+}  // namespace sicc

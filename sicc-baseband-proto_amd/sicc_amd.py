@@ -1,0 +1,296 @@
+"""ctypes binding of libsicc_amd.so -- the MI355X receive-and-decode library.
+
+This is the binding a Python caller of the reference's cuda_osd C ABI would
+write; tests and bench.py use it.  It adds no compute of
+its own: every decode goes through the HIP library, and loading fails loudly
+when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsicc_amd.so")
+DATA_DIR = os.path.join(HERE, "data")
+CCSDS_128_64 = os.path.join(DATA_DIR, "CCSDS_ldpc_n128_k64.alist")
+
+STATUS_OK = 0
+STATUS_INVALID_ARGUMENT = 1
+STATUS_INTERNAL_ERROR = 2
+
+u8p = C.POINTER(C.c_uint8)
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int)
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+f64p = C.POINTER(C.c_double)
+
+
+class StreamConfig(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("k", C.c_int), ("m", C.c_int),
+        ("max_batch_size", C.c_int), ("min_batch_size", C.c_int),
+        ("max_latency_us", C.c_ulonglong), ("arrival_ewma_alpha", C.c_double),
+        ("default_interarrival_us", C.c_ulonglong), ("bp_max_iterations", C.c_int),
+        ("bp_normalization", C.c_float), ("bp_offset", C.c_float), ("bp_damping", C.c_float),
+        ("bp_min_abs_llr_accept", C.c_float), ("osd_threads", C.c_int),
+    ]
+
+
+class OsdConfig(C.Structure):
+    _fields_ = [("n", C.c_int), ("k", C.c_int), ("max_batch_size", C.c_int), ("osd_threads", C.c_int)]
+
+
+class BpCsrConfig(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("m", C.c_int), ("max_batch_size", C.c_int), ("max_iterations", C.c_int),
+        ("normalization", C.c_float), ("offset", C.c_float), ("damping", C.c_float),
+        ("min_abs_llr_accept", C.c_float), ("nnz", C.c_int), ("schedule", C.c_int),
+    ]
+
+
+class StreamStats(C.Structure):
+    _fields_ = [
+        ("submitted_codewords", C.c_ulonglong), ("decoded_codewords", C.c_ulonglong),
+        ("bp_success_codewords", C.c_ulonglong), ("osd_fallback_codewords", C.c_ulonglong),
+        ("decoded_batches", C.c_ulonglong), ("current_pending_codewords", C.c_ulonglong),
+        ("max_pending_codewords", C.c_ulonglong), ("last_submit_timestamp_us", C.c_ulonglong),
+        ("ewma_interarrival_us", C.c_double), ("average_batch_size", C.c_double),
+        ("total_decode_ms", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    lib.cuda_osd_get_last_error.restype = C.c_char_p
+    lib.cuda_osd_get_decoded_codewords.restype = C.c_ulonglong
+    lib.cuda_osd_get_decoded_codewords.argtypes = [C.c_ulonglong]
+    lib.cuda_osd_get_total_decode_ms.restype = C.c_double
+    lib.cuda_osd_get_total_decode_ms.argtypes = [C.c_ulonglong]
+    for name in ("cuda_osd_destroy_stream_decoder", "cuda_osd_destroy_osd_decoder",
+                 "cuda_osd_destroy_bp_csr_decoder", "cuda_osd_flush"):
+        getattr(lib, name).argtypes = [C.c_ulonglong]
+    lib.sicc_decode_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int] + [C.c_void_p] * 8
+    lib.sicc_decode_host.argtypes = [C.c_ulonglong, f32p, C.c_int, u8p, u8p, f32p, u8p, u8p, i32p, f32p]
+    lib.sicc_set_osd_params.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
+    lib.sicc_get_code_info.argtypes = [C.c_ulonglong, i32p, i32p, i32p, i32p]
+    lib.sicc_get_code_csr.argtypes = [C.c_ulonglong, i32p, i32p]
+    lib.sicc_get_generator.argtypes = [C.c_ulonglong, u8p]
+    lib.sicc_encode.argtypes = [C.c_ulonglong, u8p, C.c_int, u8p]
+    lib.sicc_qc_code_csr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong, i32p, i32p, i32p, i32p]
+    lib.cuda_osd_submit_codewords.argtypes = [C.c_ulonglong, f32p, C.c_int, u64p, u64p, u32p]
+    lib.cuda_osd_submit_device_codewords.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, u64p, u64p, u32p]
+    lib.cuda_osd_poll_ready.argtypes = [C.c_ulonglong, C.c_int, u8p, u64p, u64p, u32p, u8p, i32p]
+    lib.cuda_osd_get_stats.argtypes = [C.c_ulonglong, C.POINTER(StreamStats)]
+    lib.cuda_osd_decode_bp_csr_batch.argtypes = [C.c_ulonglong, f32p, C.c_int, u8p, u8p, f64p]
+    lib.cuda_osd_decode_bp_csr_device_batch.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, u8p, u8p, f64p]
+    lib.cuda_osd_decode_osd_batch.argtypes = [C.c_ulonglong, f32p, C.c_int, u8p, f32p, f64p, f64p]
+    _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None, t):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+class SiccError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+def check(status: int):
+    if status != STATUS_OK:
+        raise SiccError(status, load().cuda_osd_get_last_error().decode(errors="replace"))
+
+
+def stream_config(n=128, k=64, max_batch=4096, min_batch=1, max_iter=50, alpha=0.75, offset=0.0,
+                  damping=0.0, min_abs_accept=0.0, latency_us=1000) -> StreamConfig:
+    return StreamConfig(n=n, k=k, m=n - k, max_batch_size=max_batch, min_batch_size=min_batch,
+                        max_latency_us=latency_us, arrival_ewma_alpha=0.15, default_interarrival_us=150,
+                        bp_max_iterations=max_iter, bp_normalization=alpha, bp_offset=offset,
+                        bp_damping=damping, bp_min_abs_llr_accept=min_abs_accept, osd_threads=256)
+
+
+class Decoder:
+    """Owns one decoder handle of the C ABI."""
+
+    def __init__(self, handle: int, kind: str):
+        self.lib = load()
+        self.handle = handle
+        self.kind = kind
+        n, k, m, e = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.sicc_get_code_info(handle, C.byref(n), C.byref(k), C.byref(m), C.byref(e)))
+        self.n, self.k, self.m, self.nnz = n.value, k.value, m.value, e.value
+
+    # -- construction --
+    @classmethod
+    def stream(cls, cfg: StreamConfig | None = None, alist: str | None = None, csr=None, builtin=False):
+        lib = load()
+        cfg = cfg or stream_config()
+        h = C.c_ulonglong(0)
+        if builtin:
+            check(lib.cuda_osd_create_stream_decoder(C.byref(cfg), C.byref(h)))
+        elif csr is not None:
+            n, m, rp, ci = csr
+            rp = np.ascontiguousarray(rp, dtype=np.int32)
+            ci = np.ascontiguousarray(ci, dtype=np.int32)
+            check(lib.sicc_create_stream_decoder_csr(C.byref(cfg), int(n), int(m), ptr(rp, i32p), ptr(ci, i32p),
+                                                     C.byref(h)))
+        else:
+            check(lib.sicc_create_stream_decoder_alist(C.byref(cfg), (alist or CCSDS_128_64).encode(), C.byref(h)))
+        return cls(h.value, "stream")
+
+    @classmethod
+    def osd_only(cls, n=128, k=64, max_batch=4096):
+        lib = load()
+        cfg = OsdConfig(n=n, k=k, max_batch_size=max_batch, osd_threads=256)
+        h = C.c_ulonglong(0)
+        check(lib.cuda_osd_create_osd_decoder(C.byref(cfg), C.byref(h)))
+        return cls(h.value, "osd")
+
+    @classmethod
+    def bp_csr(cls, n, m, row_ptr, col_ind, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, max_batch=4096,
+               schedule=-1):
+        lib = load()
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+        cfg = BpCsrConfig(n=n, m=m, max_batch_size=max_batch, max_iterations=max_iter, normalization=alpha,
+                          offset=offset, damping=damping, min_abs_llr_accept=0.0, nnz=int(rp[-1]),
+                          schedule=schedule)
+        h = C.c_ulonglong(0)
+        check(lib.cuda_osd_create_bp_csr_decoder(C.byref(cfg), ptr(rp, i32p), ptr(ci, i32p), C.byref(h)))
+        return cls(h.value, "bp")
+
+    def close(self):
+        if self.handle:
+            fn = {"stream": self.lib.cuda_osd_destroy_stream_decoder, "osd": self.lib.cuda_osd_destroy_osd_decoder,
+                  "bp": self.lib.cuda_osd_destroy_bp_csr_decoder}[self.kind]
+            check(fn(self.handle))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- code tables --
+    def csr(self):
+        rp = np.zeros(self.m + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        check(self.lib.sicc_get_code_csr(self.handle, ptr(rp, i32p), ptr(ci, i32p)))
+        return rp, ci
+
+    def generator(self):
+        g = np.zeros((self.k, self.n), np.uint8)
+        check(self.lib.sicc_get_generator(self.handle, ptr(g, u8p)))
+        return g
+
+    def encode(self, info: np.ndarray) -> np.ndarray:
+        info = np.ascontiguousarray(info, dtype=np.uint8).reshape(-1, self.k)
+        out = np.zeros((info.shape[0], self.n), np.uint8)
+        check(self.lib.sicc_encode(self.handle, ptr(info, u8p), info.shape[0], ptr(out, u8p)))
+        return out
+
+    def set_osd(self, order: int, max_flip: int = 64, max_tests: int = 4096):
+        check(self.lib.sicc_set_osd_params(self.handle, order, max_flip, max_tests))
+
+    # -- decode --
+    def decode_host(self, llr: np.ndarray, want_app=False):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        cw = llr.shape[0]
+        out = {
+            "info": np.zeros((cw, self.k), np.uint8),
+            "code": np.zeros((cw, self.n), np.uint8),
+            "app": np.zeros((cw, self.n), np.float32) if want_app else None,
+            "bp_ok": np.zeros(cw, np.uint8),
+            "used_osd": np.zeros(cw, np.uint8),
+            "iterations": np.zeros(cw, np.int32),
+            "distance": np.zeros(cw, np.float32),
+        }
+        check(self.lib.sicc_decode_host(self.handle, ptr(llr, f32p), cw, ptr(out["info"], u8p),
+                                        ptr(out["code"], u8p), ptr(out["app"], f32p), ptr(out["bp_ok"], u8p),
+                                        ptr(out["used_osd"], u8p), ptr(out["iterations"], i32p),
+                                        ptr(out["distance"], f32p)))
+        return out
+
+    def decode_device(self, d_llr: int, cw: int, d_info=None, d_code=None, d_app=None, d_ok=None, d_used=None,
+                      d_iters=None, d_dist=None, stream=None):
+        check(self.lib.sicc_decode_device(self.handle, C.c_void_p(d_llr), cw, d_info, d_code, d_app, d_ok, d_used,
+                                          d_iters, d_dist, stream))
+
+    def bp_csr_batch(self, llr: np.ndarray):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        cw = llr.shape[0]
+        code = np.zeros((cw, self.n), np.uint8)
+        ok = np.zeros(cw, np.uint8)
+        ms = C.c_double(0)
+        check(self.lib.cuda_osd_decode_bp_csr_batch(self.handle, ptr(llr, f32p), cw, ptr(code, u8p), ptr(ok, u8p),
+                                                    C.byref(ms)))
+        return code, ok, ms.value
+
+    def osd_batch(self, llr: np.ndarray):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        cw = llr.shape[0]
+        info = np.zeros((cw, self.k), np.uint8)
+        dist = np.zeros(cw, np.float32)
+        ms, mbps = C.c_double(0), C.c_double(0)
+        check(self.lib.cuda_osd_decode_osd_batch(self.handle, ptr(llr, f32p), cw, ptr(info, u8p), ptr(dist, f32p),
+                                                 C.byref(ms), C.byref(mbps)))
+        return info, dist
+
+    def submit(self, llr: np.ndarray, frame_ids=None, block_indices=None, timestamps=None):
+        llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
+        cw = llr.shape[0]
+        ts = np.ascontiguousarray(timestamps if timestamps is not None else np.zeros(cw), dtype=np.uint64)
+        ids = np.ascontiguousarray(frame_ids if frame_ids is not None else np.zeros(cw), dtype=np.uint64)
+        blk = np.ascontiguousarray(block_indices if block_indices is not None else np.arange(cw), dtype=np.uint32)
+        check(self.lib.cuda_osd_submit_codewords(self.handle, ptr(llr, f32p), cw, ptr(ts, u64p), ptr(ids, u64p),
+                                                 ptr(blk, u32p)))
+
+    def flush(self):
+        check(self.lib.cuda_osd_flush(self.handle))
+
+    def poll(self, max_cw: int):
+        info = np.zeros((max_cw, self.k), np.uint8)
+        ts = np.zeros(max_cw, np.uint64)
+        ids = np.zeros(max_cw, np.uint64)
+        blk = np.zeros(max_cw, np.uint32)
+        used = np.zeros(max_cw, np.uint8)
+        got = C.c_int(0)
+        check(self.lib.cuda_osd_poll_ready(self.handle, max_cw, ptr(info, u8p), ptr(ts, u64p), ptr(ids, u64p),
+                                           ptr(blk, u32p), ptr(used, u8p), C.byref(got)))
+        g = got.value
+        return info[:g], ids[:g], blk[:g], used[:g]
+
+    def stats(self) -> StreamStats:
+        s = StreamStats()
+        check(self.lib.cuda_osd_get_stats(self.handle, C.byref(s)))
+        return s
+
+
+def qc_code(n: int, mb: int, col_weight: int = 3, seed: int = 1):
+    """CSR of the seeded QC-LDPC code built by the library."""
+    lib = load()
+    m, e = C.c_int(), C.c_int()
+    check(lib.sicc_qc_code_csr(n, mb, col_weight, seed, C.byref(m), C.byref(e), None, None))
+    rp = np.zeros(m.value + 1, np.int32)
+    ci = np.zeros(e.value, np.int32)
+    check(lib.sicc_qc_code_csr(n, mb, col_weight, seed, None, None, ptr(rp, i32p), ptr(ci, i32p)))
+    return m.value, rp, ci

@@ -1,0 +1,159 @@
+"""GPU parity of the HIP BP / OSD decoders against the CPU oracle, through the C ABI.
+
+Integer outputs (bits, flags, iteration counts) must be identical; float
+outputs (APP LLRs, OSD distances) are bit-identical by construction and are
+compared exactly (the stated tolerance, 1e-5 relative, is not needed).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import sicc_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ccsds(ccsds_path):
+    n, m, rp, ci = oracle.alist_parse(ccsds_path)
+    return oracle.Code(n, m, rp, ci, front_info=True)
+
+
+def _frames(code, count, ebn0, seed):
+    rng = np.random.default_rng(seed)
+    info = rng.integers(0, 2, (count, code.k), dtype=np.uint8)
+    cw = code.encode(info)
+    return info, cw, oracle.awgn_llr(cw, ebn0, code.k / code.n, rng)
+
+
+@pytest.mark.parametrize("ebn0", [1.0, 2.5, 4.0])
+def test_bp_bit_exact(ccsds, ebn0):
+    print(f"[case] BP flooding CCSDS(128,64) Eb/N0={ebn0}", flush=True)
+    _, _, llr = _frames(ccsds, 3000, ebn0, 100 + int(ebn0 * 10))
+    with sa.Decoder.stream(sa.stream_config(max_iter=50, alpha=0.75)) as dec:
+        dec.set_osd(-1)
+        g = dec.decode_host(llr, want_app=True)
+    for i in range(llr.shape[0]):
+        o = ccsds.nms(llr[i], max_iter=50, alpha=0.75)
+        assert g["bp_ok"][i] == o["ok"], i
+        assert g["iterations"][i] == o["iterations"], i
+        assert np.array_equal(g["code"][i], o["hard"]), i
+        assert np.array_equal(g["app"][i], o["app"]), i
+    assert not g["used_osd"].any()
+
+
+@pytest.mark.parametrize("order,flips", [(1, 64), (2, 32), (2, 64)])
+def test_bposd_bit_exact(ccsds, order, flips):
+    print(f"[case] BP-OSD order={order} flips={flips}", flush=True)
+    _, cw, llr = _frames(ccsds, 2000, 1.5, 7 + order * 3 + flips)
+    with sa.Decoder.stream(sa.stream_config(max_iter=20, alpha=0.75)) as dec:
+        dec.set_osd(order, flips, 4096)
+        g = dec.decode_host(llr)
+    o = ccsds.bposd(llr, max_iter=20, alpha=0.75, order=order, max_flip=flips, max_tests=4096, threads=8)
+    assert o["used_osd"].sum() > 100
+    assert np.array_equal(g["used_osd"], o["used_osd"])
+    assert np.array_equal(g["bp_ok"], o["bp_ok"])
+    assert np.array_equal(g["code"], o["code"])
+    assert np.array_equal(g["info"], o["info"])
+
+
+def test_bposd_damping_offset_bit_exact(ccsds):
+    print("[case] BP-OSD damping=0.15 offset=0.15", flush=True)
+    _, _, llr = _frames(ccsds, 1500, 2.0, 55)
+    cfg = sa.stream_config(max_iter=8, alpha=0.8, offset=0.15, damping=0.15)
+    with sa.Decoder.stream(cfg) as dec:
+        dec.set_osd(1, 64, 4096)
+        g = dec.decode_host(llr)
+    o = ccsds.bposd(llr, max_iter=8, alpha=0.8, offset=0.15, damping=0.15, order=1, max_flip=64, threads=8)
+    assert np.array_equal(g["bp_ok"], o["bp_ok"])
+    assert np.array_equal(g["code"], o["code"])
+
+
+def test_osd_only_bit_exact(ccsds):
+    print("[case] OSD-only order 2 / 2081 TEPs", flush=True)
+    rng = np.random.default_rng(31)
+    _, _, llr = _frames(ccsds, 400, 1.0, 31)
+    gen = ccsds.generator()
+    with sa.Decoder.osd_only() as dec:
+        info, dist = dec.osd_batch(llr)
+    for i in range(llr.shape[0]):
+        cw, met, tests = oracle.osd(gen, llr[i], 2, 64, 4096)
+        assert tests == 2081
+        assert np.array_equal(info[i], cw[:64]), i
+        assert dist[i] == np.float32(met), i
+    del rng
+
+
+def test_generator_and_encoder_match_oracle(ccsds):
+    with sa.Decoder.stream() as dec:
+        assert (dec.n, dec.k, dec.m, dec.nnz) == (128, 64, 64, 512)
+        rp, ci = dec.csr()
+        assert np.array_equal(rp, ccsds.rp) and np.array_equal(ci, ccsds.ci)
+        assert np.array_equal(dec.generator(), ccsds.generator())
+        info = np.random.default_rng(1).integers(0, 2, (100, 64), dtype=np.uint8)
+        assert np.array_equal(dec.encode(info), ccsds.encode(info))
+
+
+def test_builtin_stream_decoder_and_poll(ccsds):
+    print("[case] stream submit/flush/poll", flush=True)
+    _, _, llr = _frames(ccsds, 700, 2.0, 77)
+    cfg = sa.stream_config(max_batch=256, min_batch=64, max_iter=20)
+    with sa.Decoder.stream(cfg, builtin=True) as dec:
+        dec.set_osd(1, 64, 4096)
+        ref = dec.decode_host(llr)
+        ids = np.arange(700) // 10
+        blk = np.arange(700) % 10
+        for s in range(0, 700, 90):
+            dec.submit(llr[s:s + 90], frame_ids=ids[s:s + 90], block_indices=blk[s:s + 90])
+        dec.flush()
+        got = {}
+        while len(got) < 700:
+            info, fid, b, used = dec.poll(128)
+            assert len(info) > 0
+            for j in range(len(info)):
+                got[int(fid[j]) * 10 + int(b[j])] = (info[j].copy(), int(used[j]))
+        st = dec.stats()
+        assert st.submitted_codewords == 700 and st.decoded_codewords == 700
+        assert st.osd_fallback_codewords == int(ref["used_osd"].sum())
+    for i in range(700):
+        assert np.array_equal(got[i][0], ref["info"][i])
+        assert got[i][1] == ref["used_osd"][i]
+
+
+def test_bp_csr_decoder_other_codes():
+    print("[case] BP CSR: CCSDS(256,128) and QC(1024, rate 1/2)", flush=True)
+    import os
+
+    path = os.path.join(sa.DATA_DIR, "CCSDS_ldpc_n256_k128.alist")
+    n, m, rp, ci = oracle.alist_parse(path)
+    m_qc, rp_qc, ci_qc = sa.qc_code(1024, 8, 3, seed=5)
+    for (n, m, rp, ci), ebn0 in [((n, m, rp, ci), 2.0), ((1024, m_qc, rp_qc, ci_qc), 1.8)]:
+        code = oracle.Code(n, m, rp, ci, front_info=True)
+        _, _, llr = _frames(code, 300, ebn0, n)
+        with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75) as dec:
+            bits, ok, _ = dec.bp_csr_batch(llr)
+        for i in range(llr.shape[0]):
+            o = code.nms(llr[i], max_iter=30, alpha=0.75)
+            assert ok[i] == o["ok"]
+            assert np.array_equal(bits[i], o["hard"])
+
+
+def test_error_paths():
+    lib = sa.load()
+    import ctypes as C
+
+    code, _ = None, None
+    assert lib.cuda_osd_flush(C.c_ulonglong(123456)) == sa.STATUS_INVALID_ARGUMENT
+    assert b"unknown handle" in lib.cuda_osd_get_last_error()
+    with sa.Decoder.stream() as dec:
+        with pytest.raises(sa.SiccError) as e:
+            dec.decode_host(np.zeros((0, 128), np.float32))
+        assert e.value.status == sa.STATUS_INVALID_ARGUMENT
+        with pytest.raises(sa.SiccError):
+            dec.set_osd(4)
+    bad = sa.stream_config(n=100, k=50)
+    with pytest.raises(sa.SiccError):
+        sa.Decoder.stream(bad, builtin=True)
+    del code
