@@ -60,6 +60,15 @@ int sicc_decode_device(unsigned long long handle, const float* device_llrs, int 
                        unsigned char* d_bp_success, unsigned char* d_used_osd, int* d_iterations,
                        float* d_distances, void* hip_stream);
 
+/* Kernel timing: with timing on, HIP events recorded on the launch stream
+ * bracket every BP and OSD kernel launch of the handle; the getter waits for
+ * them and returns summed durations (ms) and launch counts since the last
+ * enable.  No counterpart in the reference (its events are created with
+ * timing disabled, gpu_ofdm_pipeline.cu:1434-1437). */
+int sicc_set_kernel_timing(unsigned long long handle, int enable);
+int sicc_get_kernel_timing(unsigned long long handle, double* bp_ms, double* osd_ms, int* bp_launches,
+                           int* osd_launches);
+
 /* Synchronous host-buffer variant of sicc_decode_device (hard bits, soft LLRs,
  * per-frame success) for CPU callers; outputs may be null. */
 int sicc_decode_host(unsigned long long handle, const float* llrs, int codewords,

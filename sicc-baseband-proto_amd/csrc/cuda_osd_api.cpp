@@ -825,6 +825,25 @@ int sicc_decode_device(unsigned long long handle, const float* device_llrs, int 
     });
 }
 
+int sicc_set_kernel_timing(unsigned long long handle, int enable)
+{
+    return guarded([&] {
+        auto d = lookup(handle, "set_kernel_timing");
+        d->eng->set_timing(enable != 0);
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_get_kernel_timing(unsigned long long handle, double* bp_ms, double* osd_ms, int* bp_launches,
+                           int* osd_launches)
+{
+    return guarded([&] {
+        auto d = lookup(handle, "get_kernel_timing");
+        d->eng->collect_timing(bp_ms, osd_ms, bp_launches, osd_launches);
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
 int sicc_decode_host(unsigned long long handle, const float* llrs, int codewords, unsigned char* info_bits,
                      unsigned char* code_bits, float* app_llrs, unsigned char* bp_success, unsigned char* used_osd,
                      int* iterations, float* distances)

@@ -111,7 +111,9 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         io.osd_index = osd_index_.get();
         io.osd_count = osd_count_.get();
     }
+    timed_begin(0, s);
     launch_bp_flood(bpdev_, bp, io, max_waves_, s);
+    timed_end(s);
     if (use_osd) {
         OsdParams op = osd;
         op.order = osd_order;
@@ -122,7 +124,9 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         oio.code_bits = out.code;
         oio.info_bits = out.info;
         oio.distance = out.distance;
+        timed_begin(1, s);
         launch_osd(osddev_, op, oio, num_cw, s);
+        timed_end(s);
     }
 }
 
@@ -137,7 +141,81 @@ void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, cons
     oio.info_bits = out.info;
     oio.distance = out.distance;
     oio.used_osd = out.used_osd;
+    timed_begin(1, s);
     launch_osd(osddev_, osd, oio, num_cw, s);
+    timed_end(s);
+}
+
+void Engine::set_timing(bool on)
+{
+    drain_timing(true);
+    timing_ = on;
+    t_ms_[0] = t_ms_[1] = 0.0;
+    t_cnt_[0] = t_cnt_[1] = 0;
+    if (on && tpairs_.empty()) {
+        tpairs_.resize(256);
+        for (auto& p : tpairs_) {
+            SICC_HIP(hipEventCreate(&p.a));
+            SICC_HIP(hipEventCreate(&p.b));
+        }
+    }
+}
+
+void Engine::timed_begin(int kind, hipStream_t s)
+{
+    if (!timing_) return;
+    TimedPair& p = tpairs_[tnext_];
+    if (p.pending) {
+        SICC_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        SICC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        t_ms_[p.kind] += ms;
+        t_cnt_[p.kind] += 1;
+        p.pending = false;
+    }
+    p.kind = kind;
+    SICC_HIP(hipEventRecord(p.a, s));
+    tcur_ = tnext_;
+    tnext_ = (tnext_ + 1) % static_cast<int>(tpairs_.size());
+}
+
+void Engine::timed_end(hipStream_t s)
+{
+    if (!timing_ || tcur_ < 0) return;
+    TimedPair& p = tpairs_[tcur_];
+    SICC_HIP(hipEventRecord(p.b, s));
+    p.pending = true;
+    tcur_ = -1;
+}
+
+void Engine::drain_timing(bool)
+{
+    for (auto& p : tpairs_) {
+        if (!p.pending) continue;
+        SICC_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        SICC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        t_ms_[p.kind] += ms;
+        t_cnt_[p.kind] += 1;
+        p.pending = false;
+    }
+}
+
+void Engine::collect_timing(double* bp_ms, double* osd_ms, int* bp_launches, int* osd_launches)
+{
+    drain_timing(true);
+    if (bp_ms) *bp_ms = t_ms_[0];
+    if (osd_ms) *osd_ms = t_ms_[1];
+    if (bp_launches) *bp_launches = t_cnt_[0];
+    if (osd_launches) *osd_launches = t_cnt_[1];
+}
+
+Engine::~Engine()
+{
+    for (auto& p : tpairs_) {
+        if (p.a) (void)hipEventDestroy(p.a);
+        if (p.b) (void)hipEventDestroy(p.b);
+    }
 }
 
 }  // namespace sicc

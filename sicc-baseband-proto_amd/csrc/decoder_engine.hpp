@@ -8,6 +8,8 @@
 #include "kernels.hpp"
 #include "sicc_common.hpp"
 
+#include <vector>
+
 namespace sicc {
 
 struct DecodeOut {
@@ -23,6 +25,9 @@ struct DecodeOut {
 class Engine {
 public:
     explicit Engine(LdpcCode code);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
     const LdpcCode& code() const { return code_; }
     bool osd_supported() const { return osd_ok_; }
 
@@ -34,6 +39,28 @@ public:
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 
     int max_waves() const { return max_waves_; }
+
+    // Kernel timing with HIP events recorded on the launch stream around each
+    // BP and OSD launch.  collect_timing() waits for the recorded launches and
+    // returns their summed durations since the last reset.
+    void set_timing(bool on);
+    void collect_timing(double* bp_ms, double* osd_ms, int* bp_launches, int* osd_launches);
+
+private:
+    struct TimedPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        int kind = 0;  // 0 BP, 1 OSD
+        bool pending = false;
+    };
+    void timed_begin(int kind, hipStream_t s);
+    void timed_end(hipStream_t s);
+    void drain_timing(bool all);
+    bool timing_ = false;
+    std::vector<TimedPair> tpairs_;
+    int tnext_ = 0;
+    int tcur_ = -1;
+    double t_ms_[2] = {0.0, 0.0};
+    int t_cnt_[2] = {0, 0};
 
 private:
     void ensure_scratch(int num_cw);

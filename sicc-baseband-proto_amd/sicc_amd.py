@@ -84,6 +84,8 @@ def load() -> C.CDLL:
         getattr(lib, name).argtypes = [C.c_ulonglong]
     lib.sicc_decode_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int] + [C.c_void_p] * 8
     lib.sicc_decode_host.argtypes = [C.c_ulonglong, f32p, C.c_int, u8p, u8p, f32p, u8p, u8p, i32p, f32p]
+    lib.sicc_set_kernel_timing.argtypes = [C.c_ulonglong, C.c_int]
+    lib.sicc_get_kernel_timing.argtypes = [C.c_ulonglong, f64p, f64p, i32p, i32p]
     lib.sicc_set_osd_params.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
     lib.sicc_get_code_info.argtypes = [C.c_ulonglong, i32p, i32p, i32p, i32p]
     lib.sicc_get_code_csr.argtypes = [C.c_ulonglong, i32p, i32p]
@@ -234,6 +236,15 @@ class Decoder:
                       d_iters=None, d_dist=None, stream=None):
         check(self.lib.sicc_decode_device(self.handle, C.c_void_p(d_llr), cw, d_info, d_code, d_app, d_ok, d_used,
                                           d_iters, d_dist, stream))
+
+    def set_timing(self, on: bool):
+        check(self.lib.sicc_set_kernel_timing(self.handle, int(on)))
+
+    def timing(self):
+        bp, osd = C.c_double(), C.c_double()
+        nb, no = C.c_int(), C.c_int()
+        check(self.lib.sicc_get_kernel_timing(self.handle, C.byref(bp), C.byref(osd), C.byref(nb), C.byref(no)))
+        return {"bp_ms": bp.value, "osd_ms": osd.value, "bp_launches": nb.value, "osd_launches": no.value}
 
     def bp_csr_batch(self, llr: np.ndarray):
         llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
