@@ -126,8 +126,8 @@ def main() -> int:
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t1) * 1e3)
     plain_wall = time.perf_counter() - tp0
-    plain_mbps = nplain * PB * dec.k / plain_wall / 1e6
-    lat = np.array(lat)
+    plain_mbps = nplain * PB * dec.k / plain_wall / 1e6 if nplain else float("nan")
+    lat = np.array(lat if lat else [float("nan")])
 
     cpu = None
     cpu_ref = None

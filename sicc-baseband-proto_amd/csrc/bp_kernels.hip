@@ -11,12 +11,14 @@
 // decoder's, so outputs are bit-identical (built with -ffp-contract=off).
 //
 // Mapping: one wavefront decodes one codeword.  Lane l owns check rows
-// l, l+64, ... (registers hold their edge lists) and variable nodes l, l+64, ...
-// Per wave, LDS holds APP[n], the best-syndrome APP[n] and the check-to-variable
-// messages r[E] in column-major order, so both the check update (row gather of
-// APP) and the variable update (contiguous column runs of r) read LDS only.
-// The variable-to-check message q = APP - r_old is recomputed from LDS instead
-// of being stored.  HBM traffic per codeword is the LLR read and the bit/flag
+// l, l+64, ... and variable nodes l, l+64, ...  Registers hold each lane's row
+// edge lists, its rows' previous check messages (so q = APP - r_old needs no
+// stored q) and its variables' LLR / APP / best-syndrome APP.  Per wave, LDS
+// holds only what crosses lanes: APP[n] (gathered along rows) and the new
+// check messages r[E] in column-major order (summed along columns).  The check
+// update is branch-free: padded edges point at a FLT_MAX sentinel APP, min1/min2
+// use min/max, and the first-minimum test is |q| == min1 (equal values imply
+// min2 == min1).  HBM traffic per codeword is the LLR read and the bit/flag
 // writes.  Waves are persistent and stride over the batch.
 #include "kernels.hpp"
 
@@ -43,29 +45,30 @@ __global__ void zero_i32_kernel(int* p, int count)
 constexpr int kWavesPerBlock = 4;
 
 template <int MPL, int NPL, int MAXDC, bool DAMP>
-__global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
-                                                       int e_pad)
+__global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int np = 64 * NPL;
-    const int wave_floats = 2 * np + e_pad * (DAMP ? 2 : 1);
-    float* app = lds + wv * wave_floats;
-    float* best = app + np;
-    float* rcol = best + np;
-    float* qst = rcol + e_pad;
+    constexpr int np = 64 * NPL;
+    // per wave: APP[np] + one sentinel slot (FLT_MAX) for padded edges, then r[e_pad]
+    float* app = lds + wv * (np + 4 + e_pad);
+    float* rcol = app + np + 4;
 
     const int n = code.n;
     const int m = code.m;
-    const int E = code.E;
-
+    // Padded edges read the sentinel APP and write the spare message slot e_pad-1,
+    // so every row runs the same MAXDC-edge, branch-free update.
+    const int pad_edge = (np << 16) | (e_pad - 1);
     int em[MPL][MAXDC];
 #pragma unroll
     for (int s = 0; s < MPL; ++s) {
         const int r = lane + 64 * s;
 #pragma unroll
-        for (int j = 0; j < MAXDC; ++j) em[s][j] = (r < m && j < code.max_dc) ? code.row_meta[r * code.max_dc + j] : -1;
+        for (int j = 0; j < MAXDC; ++j) {
+            const int e = (r < m && j < code.max_dc) ? code.row_meta[r * code.max_dc + j] : -1;
+            em[s][j] = e >= 0 ? e : pad_edge;
+        }
     }
     int cm[NPL];
 #pragma unroll
@@ -73,89 +76,86 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
         const int v = lane + 64 * t;
         cm[t] = v < n ? code.col_meta[v] : 0;
     }
+    if (lane == 0) app[np] = FLT_MAX;
 
     const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
     const int nw = gridDim.x * (blockDim.x >> 6);
     const float alpha = p.alpha;
+    const float off = p.offset;
+    const float damp = p.damping;
 
     for (int cw = gw; cw < io.num_cw; cw += nw) {
         const float* L = io.llr + static_cast<size_t>(cw) * n;
-        float lr[NPL];
+        float lr[NPL], appr[NPL], bestr[NPL];
 #pragma unroll
         for (int t = 0; t < NPL; ++t) {
             const int v = lane + 64 * t;
             lr[t] = v < n ? L[v] : 0.0f;
+            appr[t] = lr[t];
+            bestr[t] = lr[t];
             app[v] = lr[t];
         }
-        for (int i = lane; i < E; i += 64) rcol[i] = 0.0f;
+        float rold[MPL][MAXDC];
+        float qold[MPL][MAXDC];
+#pragma unroll
+        for (int s = 0; s < MPL; ++s)
+#pragma unroll
+            for (int j = 0; j < MAXDC; ++j) {
+                rold[s][j] = 0.0f;
+                qold[s][j] = 0.0f;
+            }
         wave_lds_sync();
 
         int best_w = 0;
-        bool best_is_channel = true;
         bool ok = false;
         int it = 0;
         for (;;) {
-            // ---- check-node update + syndrome of the current hard decision ----
+            // ---- check-node update; its APP gather also yields the syndrome ----
             int synw = 0;
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
-                int hp = 0;
-                int sp = 0;
-                float m1 = FLT_MAX;
-                float m2 = FLT_MAX;
-                int i1 = -1;
+                unsigned hp = 0u, sp = 0u;
+                float m1 = FLT_MAX, m2 = FLT_MAX;
                 float qv[MAXDC];
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
-                    const int e = em[s][j];
-                    qv[j] = 0.0f;
-                    if (e >= 0) {
-                        const int col = e >> 16;
-                        const int slot = e & 0xffff;
-                        const float a = app[col];
-                        hp ^= (a < 0.0f) ? 1 : 0;
-                        float q = a - rcol[slot];
-                        if (DAMP) {
-                            if (it > 0) q = (1.0f - p.damping) * q + p.damping * qst[slot];
-                            qst[slot] = q;
-                        }
-                        qv[j] = q;
-                        sp ^= (q < 0.0f) ? 1 : 0;
-                        const float av = fabsf(q);
-                        if (av < m1) {
-                            m2 = m1;
-                            m1 = av;
-                            i1 = j;
-                        } else if (av < m2) {
-                            m2 = av;
-                        }
+                    const float a = app[em[s][j] >> 16];
+                    hp ^= __float_as_uint(a + 0.0f);  // +0.0f maps -0 to +0: sign bit == (a < 0)
+                    float q = a - rold[s][j];
+                    if (DAMP) {
+                        if (it > 0) q = (1.0f - damp) * q + damp * qold[s][j];
+                        qold[s][j] = q;
                     }
+                    q = q + 0.0f;
+                    qv[j] = q;
+                    sp ^= __float_as_uint(q);
+                    const float av = fabsf(q);
+                    m2 = fminf(m2, fmaxf(m1, av));
+                    m1 = fminf(m1, av);
                 }
-                const bool row_valid = em[s][0] >= 0;
-                synw += __popcll(__ballot(row_valid && hp));
-                // Writing r_new right away is safe: each slot belongs to one row,
-                // and nothing reads r before the variable update below.
+                synw += __popcll(__ballot((hp >> 31) != 0u));
+                // The first minimum edge takes min2; any other edge equal to min1
+                // implies min2 == min1, so |q| == min1 selects the same magnitude.
+                float a1 = m1, a2 = m2;
+                if (off != 0.0f) {
+                    a1 = fmaxf(a1 - off, 0.0f);
+                    a2 = fmaxf(a2 - off, 0.0f);
+                }
+                a1 = alpha * a1;
+                a2 = alpha * a2;
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
-                    const int e = em[s][j];
-                    if (e >= 0) {
-                        const bool neg = (sp ^ (qv[j] < 0.0f ? 1 : 0)) != 0;
-                        float mag = (j == i1) ? m2 : m1;
-                        if (p.offset != 0.0f) mag = fmaxf(mag - p.offset, 0.0f);
-                        const float sf = neg ? -1.0f : 1.0f;
-                        rcol[e & 0xffff] = alpha * sf * mag;
-                    }
+                    const float mag = (fabsf(qv[j]) == m1) ? a2 : a1;
+                    const float r = __uint_as_float(__float_as_uint(mag) | ((sp ^ __float_as_uint(qv[j])) & 0x80000000u));
+                    rold[s][j] = r;
+                    rcol[em[s][j] & 0xffff] = r;
                 }
             }
             if (it > 0) {
                 if (synw < best_w || synw == 0) {
                     best_w = synw;
-                    best_is_channel = false;
 #pragma unroll
-                    for (int t = 0; t < NPL; ++t) {
-                        const int v = lane + 64 * t;
-                        best[v] = app[v];
-                    }
+                    for (int t = 0; t < NPL; ++t) bestr[t] = appr[t];
                 }
                 if (synw == 0) {
                     ok = true;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
             }
             if (it == p.max_iter) break;
             wave_lds_sync();
-            // ---- variable-node update: APP = LLR + sum r (increasing row) ----
+            // ---- variable-node update: APP = LLR + sum r, increasing row order ----
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
                 const int v = lane + 64 * t;
@@ -175,6 +175,7 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
                     const int st = cm[t] >> 8;
                     const int dg = cm[t] & 255;
                     for (int j = 0; j < dg; ++j) a += rcol[st + j];
+                    appr[t] = a;
                     app[v] = a;
                 }
             }
@@ -187,11 +188,9 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
         if (ok && p.min_abs_accept > 0.0f) {
             float mn = FLT_MAX;
 #pragma unroll
-            for (int t = 0; t < NPL; ++t) {
-                const int v = lane + 64 * t;
-                if (v < n) mn = fminf(mn, fabsf(app[v]));
-            }
-            for (int off = 32; off > 0; off >>= 1) mn = fminf(mn, __shfl_xor(mn, off));
+            for (int t = 0; t < NPL; ++t)
+                if (lane + 64 * t < n) mn = fminf(mn, fabsf(appr[t]));
+            for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
             accept = mn >= p.min_abs_accept;
         }
         if (io.code_bits || io.app) {
@@ -199,9 +198,8 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
             for (int t = 0; t < NPL; ++t) {
                 const int v = lane + 64 * t;
                 if (v < n) {
-                    const float a = app[v];
-                    if (io.code_bits) io.code_bits[static_cast<size_t>(cw) * n + v] = a < 0.0f ? 1 : 0;
-                    if (io.app) io.app[static_cast<size_t>(cw) * n + v] = a;
+                    if (io.code_bits) io.code_bits[static_cast<size_t>(cw) * n + v] = appr[t] < 0.0f ? 1 : 0;
+                    if (io.app) io.app[static_cast<size_t>(cw) * n + v] = appr[t];
                 }
             }
         }
@@ -221,10 +219,11 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
             if (lane == 0) slot = atomicAdd(io.osd_count, 1);
             slot = __shfl(slot, 0);
             float* dst = io.osd_llr + static_cast<size_t>(slot) * n;
+            // a converged-but-rejected frame hands over its final APP (== best)
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
                 const int v = lane + 64 * t;
-                if (v < n) dst[v] = best_is_channel ? lr[t] : best[v];
+                if (v < n) dst[v] = bestr[t];
             }
             if (lane == 0) io.osd_index[slot] = cw;
         }
@@ -235,9 +234,9 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
 template <int MPL, int NPL, int MAXDC>
 void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, hipStream_t s)
 {
-    const int e_pad = (code.E + 63) & ~63;
+    const int e_pad = (code.E + 1 + 63) & ~63;  // one spare slot for padded edges
     const bool damp = p.damping != 0.0f;
-    const size_t wave_bytes = sizeof(float) * static_cast<size_t>(2 * 64 * NPL + e_pad * (damp ? 2 : 1));
+    const size_t wave_bytes = sizeof(float) * static_cast<size_t>(64 * NPL + 4 + e_pad);
     // At most 64 KiB of LDS per workgroup; fewer waves per group for big codes.
     int wpb = static_cast<int>((64 * 1024) / wave_bytes);
     if (wpb > kWavesPerBlock) wpb = kWavesPerBlock;
