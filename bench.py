@@ -51,6 +51,8 @@ def main() -> int:
     ap.add_argument("--plain-batches", type=int, default=400)
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one device copy of the LLR buffer (known bytes) to scale HBM counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this decoder runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -82,6 +84,11 @@ def main() -> int:
         dec.decode_device(d_llr.data_ptr(), B, d_info=d_info.data_ptr(), d_ok=d_ok.data_ptr(),
                           d_used=d_used.data_ptr(), d_iters=d_it.data_ptr(), stream=sh)
 
+    if a.copy_calibration:
+        calib = torch.empty_like(d_llr)
+        calib.copy_(d_llr)  # reads and writes B*n*4 bytes
+        torch.cuda.synchronize()
+        del calib
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
