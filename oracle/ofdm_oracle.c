@@ -1,0 +1,263 @@
+/*
+ * ofdm_oracle.c -- CPU restatement of the reference OFDM receive path.
+ * TEST INFRASTRUCTURE ONLY (checker for tests/, smoke and bench.py's CPU leg).
+ *
+ * Sources restated (paths relative to the reference repository root):
+ *   LINK = source/cpp_cuda/uhd_cpp/src/uhd_ldpc_ofdm_link.cpp
+ *     fft_inplace            LINK:3788-3827
+ *     fft_to_shifted_grid    LINK:3845-3859
+ *     find_frame             LINK:5123-5212
+ *     decode_frame (demod)   LINK:5331-5521, known-noise-variance branch
+ *     symbol_to_llr_maxlog   LINK:2552-2568
+ *
+ * Complex arithmetic is written out as std::complex performs it in the
+ * reference build: products (ac - bd, ad + bc), division by a real scalar per
+ * component, std::norm(z) = re*re + im*im in the element type.
+ * IQ buffers are interleaved float pairs (re, im).
+ */
+#include "sicc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double kPi = 3.141592653589793238462643383279502884;
+
+/* std::norm of a complex<float> in libstdc++: re*re + im*im in float. */
+static float normf(float re, float im) { return re * re + im * im; }
+
+/* LINK:3788-3827: in-place radix-2, bit reversal then butterflies with the
+ * twiddle advanced by repeated multiplication w *= wlen; inverse scales 1/n. */
+void oracle_fft(double* re, double* im, int n, int inverse)
+{
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i];
+            re[i] = re[j];
+            re[j] = t;
+            t = im[i];
+            im[i] = im[j];
+            im[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const double ang = 2.0 * kPi / (double)len * (inverse ? 1.0 : -1.0);
+        const double wr = cos(ang), wi = sin(ang);
+        for (int i = 0; i < n; i += len) {
+            double cr = 1.0, ci = 0.0;
+            for (int j = 0; j < len / 2; ++j) {
+                const double ur = re[i + j], ui = im[i + j];
+                const double xr = re[i + j + len / 2], xi = im[i + j + len / 2];
+                const double vr = xr * cr - xi * ci;
+                const double vi = xr * ci + xi * cr;
+                re[i + j] = ur + vr;
+                im[i + j] = ui + vi;
+                re[i + j + len / 2] = ur - vr;
+                im[i + j + len / 2] = ui - vi;
+                const double nr = cr * wr - ci * wi;
+                const double ni = cr * wi + ci * wr;
+                cr = nr;
+                ci = ni;
+            }
+        }
+    }
+    if (inverse) {
+        const double s = 1.0 / (double)n;
+        for (int i = 0; i < n; ++i) {
+            re[i] *= s;
+            im[i] *= s;
+        }
+    }
+}
+
+/* LINK:5123-5212 with an optional bound on the coarse search (as the
+ * reference's max-search-samples option, LINK:5214-5246): candidates d in
+ * [0, min(count - pre_len, max_search)] (max_search <= 0: all of them),
+ * refinement over [best - radius, best + radius] clamped to
+ * [0, count - pre_len].
+ * Returns 1 when a frame passes both thresholds. */
+int oracle_find_frame(const float* iq, int count, int half_len, const float* preamble, int pre_len,
+                      int refine_radius, int max_search, double threshold, double fs, int* start_out,
+                      double* peak_out, double* cfo_out, double* coarse_out)
+{
+    const int L = half_len;
+    *start_out = -1;
+    *peak_out = 0.0;
+    *cfo_out = 0.0;
+    if (coarse_out) *coarse_out = 0.0;
+    if (count < pre_len || pre_len != 2 * L) return 0;
+    const int last = count - pre_len; /* highest candidate */
+    const int limit = (max_search > 0 && max_search < last) ? max_search : last;
+    if (refine_radius < 0) refine_radius = 0;
+    double pr = 0.0, pi = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int i = 0; i < L; ++i) {
+        const double ar = iq[2 * i], ai = iq[2 * i + 1];
+        const double br = iq[2 * (i + L)], bi = iq[2 * (i + L) + 1];
+        pr += ar * br + ai * bi; /* conj(a) * b */
+        pi += ar * bi - ai * br;
+        e1 += (double)normf(iq[2 * i], iq[2 * i + 1]);
+        e2 += (double)normf(iq[2 * (i + L)], iq[2 * (i + L) + 1]);
+    }
+    int best = 0;
+    double best_metric = 0.0;
+    for (int d = 0; d <= limit; ++d) {
+        const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
+        const double metric = (pr * pr + pi * pi) / den;
+        if (metric > best_metric) {
+            best_metric = metric;
+            best = d;
+        }
+        if (d + pre_len >= count) break;
+        const double x0r = iq[2 * d], x0i = iq[2 * d + 1];
+        const double x1r = iq[2 * (d + L)], x1i = iq[2 * (d + L) + 1];
+        const double x2r = iq[2 * (d + pre_len)], x2i = iq[2 * (d + pre_len) + 1];
+        /* p += conj(x1) x2 - conj(x0) x1 */
+        pr += (x1r * x2r + x1i * x2i) - (x0r * x1r + x0i * x1i);
+        pi += (x1r * x2i - x1i * x2r) - (x0r * x1i - x0i * x1r);
+        /* float difference of float norms, then accumulated in double */
+        e1 += (double)(normf(iq[2 * (d + L)], iq[2 * (d + L) + 1]) - normf(iq[2 * d], iq[2 * d + 1]));
+        e2 += (double)(normf(iq[2 * (d + pre_len)], iq[2 * (d + pre_len) + 1]) -
+                       normf(iq[2 * (d + L)], iq[2 * (d + L) + 1]));
+    }
+    if (coarse_out) *coarse_out = best_metric;
+    if (best_metric < threshold) return 0;
+    const int lo = best > refine_radius ? best - refine_radius : 0;
+    const int hi = (best + refine_radius < last) ? best + refine_radius : last;
+    double pre_energy = 0.0;
+    for (int i = 0; i < pre_len; ++i) pre_energy += (double)normf(preamble[2 * i], preamble[2 * i + 1]);
+    double best_corr_metric = 0.0;
+    int best_corr = best;
+    for (int d = lo; d <= hi; ++d) {
+        double cr = 0.0, ci = 0.0, eng = 0.0;
+        for (int i = 0; i < pre_len; ++i) {
+            const double yr = iq[2 * (d + i)], yi = iq[2 * (d + i) + 1];
+            const double qr = preamble[2 * i], qi = preamble[2 * i + 1];
+            cr += yr * qr + yi * qi; /* y * conj(p) */
+            ci += yi * qr - yr * qi;
+            eng += (double)normf(iq[2 * (d + i)], iq[2 * (d + i) + 1]);
+        }
+        const double den = eng * pre_energy > 1e-18 ? eng * pre_energy : 1e-18;
+        const double m = (cr * cr + ci * ci) / den;
+        if (m > best_corr_metric) {
+            best_corr_metric = m;
+            best_corr = d;
+        }
+    }
+    *peak_out = best_corr_metric;
+    if (best_corr_metric < threshold) return 0;
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < L; ++i) {
+        const double xr = iq[2 * (best_corr + i)], xi = iq[2 * (best_corr + i) + 1];
+        const double yr = iq[2 * (best_corr + i + L)], yi = iq[2 * (best_corr + i + L) + 1];
+        ar += xr * yr + xi * yi;
+        ai += xr * yi - xi * yr;
+    }
+    *start_out = best_corr;
+    *cfo_out = atan2(ai, ar) * fs / (2.0 * kPi * (double)L);
+    return 1;
+}
+
+/* LINK:5331-5521 demodulation with a known noise variance: CFO derotation with
+ * the phase origin at the frame start, per-symbol FP64 FFT to the shifted grid,
+ * LS pilot estimate, linear interpolation between pilot symbols, equalization
+ * eq = y conj(h) / (|h|^2 + s2), s2_eq = s2 / |h|^2, max-log LLR
+ * (d1 - d0) / s2_eq over the constellation (LINK:2552-2568).  LLR order:
+ * ((data symbol * active_sc) + subcarrier) * bits + bit. */
+void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int num_symbols, int active_sc,
+                        const int* used_shifted, int n_pilot, const int* pilot_sym, const float* pilot_grid,
+                        int n_data, const int* data_sym, int bits_per_symbol, const float* constellation,
+                        double cfo_hz, double fs, double noise_var, float* llr_out)
+{
+    const int sym_len = nfft + cp;
+    const int frame_len = pre_len + num_symbols * sym_len;
+    float* y = (float*)malloc(sizeof(float) * 2 * (size_t)num_symbols * active_sc);
+    double* re = (double*)malloc(sizeof(double) * (size_t)nfft);
+    double* im = (double*)malloc(sizeof(double) * (size_t)nfft);
+    float* rot = (float*)malloc(sizeof(float) * 2 * (size_t)frame_len);
+    for (int n = 0; n < frame_len; ++n) {
+        const double a = -2.0 * kPi * cfo_hz * (double)n / fs;
+        const float c = (float)cos(a), s = (float)sin(a);
+        const float xr = frame[2 * n], xi = frame[2 * n + 1];
+        rot[2 * n] = xr * c - xi * s;
+        rot[2 * n + 1] = xr * s + xi * c;
+    }
+    for (int sym = 0; sym < num_symbols; ++sym) {
+        const float* td = rot + 2 * (size_t)(pre_len + sym * sym_len + cp);
+        for (int i = 0; i < nfft; ++i) {
+            re[i] = td[2 * i];
+            im[i] = td[2 * i + 1];
+        }
+        oracle_fft(re, im, nfft, 0);
+        for (int sc = 0; sc < active_sc; ++sc) {
+            const int nat = (used_shifted[sc] + nfft / 2) % nfft;
+            y[2 * ((size_t)sym * active_sc + sc)] = (float)re[nat];
+            y[2 * ((size_t)sym * active_sc + sc) + 1] = (float)im[nat];
+        }
+    }
+    float* hp = (float*)malloc(sizeof(float) * 2 * (size_t)(n_pilot > 0 ? n_pilot : 1) * active_sc);
+    for (int p = 0; p < n_pilot; ++p) {
+        const int sym = pilot_sym[p];
+        for (int sc = 0; sc < active_sc; ++sc) {
+            const float pr = pilot_grid[2 * ((size_t)p * active_sc + sc)];
+            const float pi = pilot_grid[2 * ((size_t)p * active_sc + sc) + 1];
+            float den = normf(pr, pi);
+            if (den < 1e-9f) den = 1e-9f;
+            const float yr = y[2 * ((size_t)sym * active_sc + sc)], yi = y[2 * ((size_t)sym * active_sc + sc) + 1];
+            /* y * conj(p) / den */
+            const float tr = yr * pr - yi * (-pi);
+            const float ti = yr * (-pi) + yi * pr;
+            hp[2 * ((size_t)p * active_sc + sc)] = tr / den;
+            hp[2 * ((size_t)p * active_sc + sc) + 1] = ti / den;
+        }
+    }
+    const int M = 1 << bits_per_symbol;
+    if (noise_var < 1e-8) noise_var = 1e-8;
+    const float nv = (float)noise_var;
+    for (int di = 0; di < n_data; ++di) {
+        const int sym = data_sym[di];
+        int hi = 0;
+        while (hi + 1 < n_pilot && sym > pilot_sym[hi + 1]) ++hi;
+        int p0 = hi, p1 = (hi + 1 < n_pilot - 1) ? hi + 1 : n_pilot - 1;
+        if (sym <= pilot_sym[0]) p0 = p1 = 0;
+        else if (sym >= pilot_sym[n_pilot - 1]) p0 = p1 = n_pilot - 1;
+        double alpha = 0.0;
+        if (p0 != p1) alpha = (double)(sym - pilot_sym[p0]) / (double)(pilot_sym[p1] - pilot_sym[p0]);
+        const float w0 = (float)(1.0 - alpha), w1 = (float)alpha;
+        for (int sc = 0; sc < active_sc; ++sc) {
+            const float h0r = hp[2 * ((size_t)p0 * active_sc + sc)], h0i = hp[2 * ((size_t)p0 * active_sc + sc) + 1];
+            const float h1r = hp[2 * ((size_t)p1 * active_sc + sc)], h1i = hp[2 * ((size_t)p1 * active_sc + sc) + 1];
+            const float hr = h0r * w0 + h1r * w1;
+            const float hi_ = h0i * w0 + h1i * w1;
+            float h2 = normf(hr, hi_);
+            if (h2 < 1e-9f) h2 = 1e-9f;
+            const float yr = y[2 * ((size_t)sym * active_sc + sc)], yi = y[2 * ((size_t)sym * active_sc + sc) + 1];
+            const float tr = yr * hr - yi * (-hi_);
+            const float ti = yr * (-hi_) + yi * hr;
+            const float den = h2 + nv;
+            const float er = tr / den, ei = ti / den;
+            double s2 = noise_var / (double)h2;
+            if (s2 < 1e-8) s2 = 1e-8;
+            float* dst = llr_out + ((size_t)di * active_sc + sc) * bits_per_symbol;
+            for (int b = 0; b < bits_per_symbol; ++b) {
+                float d0 = INFINITY, d1 = INFINITY;
+                for (int i = 0; i < M; ++i) {
+                    const float d2 = normf(er - constellation[2 * i], ei - constellation[2 * i + 1]);
+                    if (((i >> (bits_per_symbol - 1 - b)) & 1) == 0) {
+                        if (d2 < d0) d0 = d2;
+                    } else if (d2 < d1) {
+                        d1 = d2;
+                    }
+                }
+                dst[b] = (float)((double)(d1 - d0) / s2);
+            }
+        }
+    }
+    free(y);
+    free(re);
+    free(im);
+    free(rot);
+    free(hp);
+}

@@ -142,3 +142,44 @@ def awgn_llr(codewords: np.ndarray, ebn0_db: float, rate: float, rng: np.random.
     x = 1.0 - 2.0 * codewords.astype(np.float64)
     y = x + np.sqrt(sigma2) * rng.standard_normal(codewords.shape)
     return (2.0 / sigma2 * y).astype(np.float32)
+
+
+# ---------------------------------------------------------------- OFDM ----
+def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """LINK:3788 radix-2 FFT on a complex vector (FP64)."""
+    re = np.ascontiguousarray(x.real, np.float64)
+    im = np.ascontiguousarray(x.imag, np.float64)
+    load().oracle_fft(_p(re, f64p), _p(im, f64p), len(x), int(inverse))
+    return re + 1j * im
+
+
+def _iq(x: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, np.complex64).view(np.float32))
+
+
+def find_frame(buf: np.ndarray, half_len: int, preamble: np.ndarray, refine_radius: int, threshold: float,
+               fs: float, max_search: int = 0):
+    """LINK:5123 find_frame on a complex64 buffer -> (found, start, peak, cfo_hz, coarse_metric)."""
+    iq = _iq(buf)
+    pre = _iq(preamble)
+    start, peak, cfo, coarse = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+    ok = load().oracle_find_frame(_p(iq, f32p), len(buf), half_len, _p(pre, f32p), len(preamble), refine_radius,
+                                  max_search, C.c_double(threshold), C.c_double(fs), C.byref(start), C.byref(peak),
+                                  C.byref(cfo), C.byref(coarse))
+    return bool(ok), start.value, peak.value, cfo.value, coarse.value
+
+
+def demod_frame(frame: np.ndarray, phy, cfo_hz: float, noise_var: float) -> np.ndarray:
+    """LINK:5331-5521 demodulation of one frame (starting at the preamble) to LLRs."""
+    iq = _iq(frame)
+    used = np.ascontiguousarray(phy.used_shifted, np.int32)
+    ps = np.ascontiguousarray(phy.pilot_symbols, np.int32)
+    ds = np.ascontiguousarray(phy.data_symbols, np.int32)
+    pg = _iq(phy.pilot_grid.reshape(-1))
+    const = _iq(phy.constellation)
+    out = np.zeros(len(ds) * phy.active_sc * phy.bits_per_symbol, np.float32)
+    load().oracle_demod_frame(_p(iq, f32p), phy.nfft, phy.cp, phy.pre_len, phy.num_symbols, phy.active_sc,
+                              _p(used, i32p), len(ps), _p(ps, i32p), _p(pg, f32p), len(ds), _p(ds, i32p),
+                              phy.bits_per_symbol, _p(const, f32p), C.c_double(cfo_hz), C.c_double(phy.fs),
+                              C.c_double(noise_var), _p(out, f32p))
+    return out

@@ -63,3 +63,71 @@ class RefCode:
         load().ref_min_sum(_p(llr, f32p), c, max_iter, C.c_double(norm), _p(code, u8p), _p(info, u8p), _p(ok, u8p),
                            _p(it, i32p))
         return code, info, ok, it
+
+
+class RefPhy:
+    """The reference PhyConfig (build_phy_config, LINK:3971) and its CPU receive functions."""
+
+    def __init__(self, nfft=1024, cp=72, num_symbols=14, active_sc=600, pre_half_len=512, modulation="qpsk",
+                 pilot_period=14, rate=15.36e6, sync_threshold=0.45, amplitude=0.7, test_seed=20260418):
+        lib = load()
+        if lib.ref_phy_setup(nfft, cp, num_symbols, active_sc, pre_half_len, modulation.encode(), pilot_period,
+                             C.c_double(rate), C.c_double(sync_threshold), C.c_double(amplitude), test_seed) != 0:
+            raise RuntimeError(lib.ref_last_error().decode())
+        fl, cb, npil, nd, bps, pl = (C.c_int() for _ in range(6))
+        lib.ref_phy_dims(C.byref(fl), C.byref(cb), C.byref(npil), C.byref(nd), C.byref(bps), C.byref(pl))
+        self.frame_len, self.coded_bits, self.bits_per_symbol = fl.value, cb.value, bps.value
+        self.nfft, self.cp, self.num_symbols, self.active_sc = nfft, cp, num_symbols, active_sc
+        self.pre_half_len, self.fs, self.sync_threshold = pre_half_len, rate, sync_threshold
+        pre = np.zeros(2 * pl.value, np.float32)
+        used = np.zeros(active_sc, np.int32)
+        ps = np.zeros(npil.value, np.int32)
+        ds = np.zeros(nd.value, np.int32)
+        pg = np.zeros(2 * npil.value * active_sc, np.float32)
+        const = np.zeros(2 << bps.value, np.float32)
+        lib.ref_phy_tables(_p(pre, f32p), _p(used, i32p), _p(ps, i32p), _p(ds, i32p), _p(pg, f32p),
+                           _p(const, f32p))
+        self.preamble = pre.view(np.complex64)
+        self.pre_len = len(self.preamble)
+        self.used_shifted = used
+        self.pilot_symbols = ps
+        self.data_symbols = ds
+        self.pilot_grid = pg.view(np.complex64).reshape(npil.value, active_sc)
+        self.constellation = const.view(np.complex64)
+
+    def build_tx_frame(self, info_bits: np.ndarray) -> np.ndarray:
+        info = np.ascontiguousarray(info_bits, np.uint8)
+        out = np.zeros(2 * self.frame_len, np.float32)
+        if load().ref_build_tx_frame(_p(info, u8p), _p(out, f32p)) < 0:
+            raise RuntimeError(load().ref_last_error().decode())
+        return out.view(np.complex64)
+
+    def find_frame(self, buf: np.ndarray):
+        iq = np.ascontiguousarray(np.asarray(buf, np.complex64).view(np.float32))
+        s, p, c = C.c_int(), C.c_double(), C.c_double()
+        ok = load().ref_find_frame(_p(iq, f32p), len(buf), C.byref(s), C.byref(p), C.byref(c))
+        return bool(ok), s.value, p.value, c.value
+
+    def fft_shifted(self, sym: np.ndarray) -> np.ndarray:
+        iq = np.ascontiguousarray(np.asarray(sym, np.complex64).view(np.float32))
+        out = np.zeros(2 * len(sym), np.float32)
+        load().ref_fft_shifted(_p(iq, f32p), len(sym), _p(out, f32p))
+        return out.view(np.complex64)
+
+    def symbol_llr(self, eq: np.ndarray, sigma2: np.ndarray) -> np.ndarray:
+        iq = np.ascontiguousarray(np.asarray(eq, np.complex64).view(np.float32))
+        s2 = np.ascontiguousarray(sigma2, np.float64)
+        out = np.zeros(len(eq) * self.bits_per_symbol, np.float32)
+        load().ref_symbol_llr(_p(iq, f32p), len(eq), _p(s2, f64p), _p(out, f32p))
+        return out
+
+    def decode_frame(self, frame: np.ndarray, cfo_hz: float, noise_var: float, max_iter: int, norm: float,
+                     info_len: int):
+        iq = np.ascontiguousarray(np.asarray(frame, np.complex64).view(np.float32))
+        info = np.zeros(info_len, np.uint8)
+        ok, snr = C.c_int(), C.c_double()
+        r = load().ref_decode_frame(_p(iq, f32p), C.c_double(cfo_hz), C.c_double(noise_var), max_iter,
+                                    C.c_double(norm), _p(info, u8p), C.byref(ok), C.byref(snr))
+        if r < 0:
+            raise RuntimeError(load().ref_last_error().decode())
+        return info[:r], bool(ok.value)

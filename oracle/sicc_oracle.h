@@ -67,14 +67,13 @@ int oracle_bposd_batch(int count, int n, int m, int k, const int* row_ptr, const
 /* ---- OFDM receive (LINK:3788-3860, LINK:5123-5212, LINK:5331-5521) ---- */
 void oracle_fft(double* re, double* im, int n, int inverse);
 int oracle_find_frame(const float* iq, int count, int half_len, const float* preamble,
-                      int pre_len, int refine_radius, double threshold, double fs,
+                      int pre_len, int refine_radius, int max_search, double threshold, double fs,
                       int* start_out, double* peak_out, double* cfo_out, double* coarse_out);
 void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int num_symbols,
                         int active_sc, const int* used_shifted, int n_pilot, const int* pilot_sym,
                         const float* pilot_grid, int n_data, const int* data_sym,
-                        const int* interp_p0, const int* interp_p1, const float* interp_alpha,
-                        int bits_per_symbol, double cfo_hz, double fs, double noise_var,
-                        long phase_origin, float* llr_out);
+                        int bits_per_symbol, const float* constellation, double cfo_hz, double fs,
+                        double noise_var, float* llr_out);
 
 #ifdef __cplusplus
 }
