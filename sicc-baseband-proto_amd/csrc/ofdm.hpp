@@ -1,0 +1,66 @@
+// ofdm.hpp -- device argument blocks and launchers of the OFDM receive kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sicc {
+
+// Per-frame synchronization result (device).  start < 0: no frame passed the
+// threshold.  cfo_hz_d keeps the double estimate the demodulator uses.
+struct OfdmSyncResult {
+    int start;
+    float metric;         // refined peak (full-preamble normalized correlation)
+    float cfo_hz;
+    float coarse_metric;  // Schmidl-Cox metric at the coarse argmax
+    double cfo_hz_d;
+};
+
+struct OfdmSyncArgs {
+    const float2* rx = nullptr;  // [frames][span]
+    int span = 0;
+    int half_len = 0;
+    int max_search = 0;          // coarse window [0, max_search]; <= 0: whole row
+    int radius = 16;             // refinement half-width
+    double threshold = 0.45;
+    double fs = 15.36e6;
+    const float2* preamble = nullptr;  // [2 * half_len]
+    double pre_energy = 0.0;           // sum of |preamble|^2 (float norms, double sum)
+    OfdmSyncResult* out = nullptr;     // [frames]
+    int window_in_lds = 1;             // set by the launcher
+};
+
+struct OfdmDemodArgs {
+    const float2* rx = nullptr;  // [frames][span], frame f starts at sync[f].start
+    int span = 0;
+    const OfdmSyncResult* sync = nullptr;
+    int nfft = 1024;
+    int cp = 72;
+    int preamble_len = 1024;
+    int active_sc = 600;
+    int bits_per_symbol = 4;
+    int n_pilot = 1;
+    int n_data = 13;
+    const int* pilot_symbols = nullptr;  // [n_pilot]
+    const int* data_symbols = nullptr;   // [n_data]
+    const int* used_natural = nullptr;   // [active_sc] natural FFT bin of each subcarrier
+    const float2* pilot_grid = nullptr;  // [n_pilot][active_sc]
+    const int* interp_p0 = nullptr;      // [n_data]
+    const int* interp_p1 = nullptr;
+    const float* interp_w0 = nullptr;    // (float)(1 - alpha)
+    const float* interp_w1 = nullptr;    // (float)alpha
+    const float2* constellation = nullptr;  // [2^bits]
+    const double2* twiddle = nullptr;       // [nfft] exp(-2 pi i m / nfft)
+    float noise_var = 1e-3f;
+    double noise_var_d = 1e-3;
+    double fs = 15.36e6;
+    int use_pilots = 1;
+    float* llr = nullptr;  // [frames][llr_per_frame]
+    int llr_per_frame = 0;
+};
+
+void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s);
+void launch_ofdm_demod(const OfdmDemodArgs& a, int frames, hipStream_t s);
+
+}  // namespace sicc

@@ -1,0 +1,42 @@
+// ofdm_rx.hpp -- host object owning the device tables of one OFDM receiver.
+#pragma once
+
+#include "../../include/sicc_ofdm_c_api.h"
+#include "ofdm.hpp"
+#include "sicc_common.hpp"
+
+#include <vector>
+
+namespace sicc {
+
+class OfdmRx {
+public:
+    // Tables: preamble [preamble_len] complex, used_shifted [active_sc],
+    // pilot_symbols [n_pilot], pilot_grid [n_pilot][active_sc] complex.
+    OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted, int n_pilot,
+           const int* pilot_symbols, const float2* pilot_grid);
+
+    const sicc_ofdm_config_t& config() const { return cfg_; }
+    int frame_samples() const { return cfg_.preamble_len + cfg_.num_symbols * (cfg_.nfft + cfg_.cp); }
+    int llr_per_frame() const { return cfg_.blocks_per_frame * cfg_.code_n; }
+
+    void sync(const float2* d_rx, int frames, int span, OfdmSyncResult* d_out, int max_search, hipStream_t s) const;
+    void demod(const float2* d_rx, int frames, int span, const OfdmSyncResult* d_sync, float* d_llr,
+               hipStream_t s) const;
+
+private:
+    sicc_ofdm_config_t cfg_;
+    int n_pilot_ = 0;
+    int n_data_ = 0;
+    double pre_energy_ = 0.0;
+    DevBuf<float2> preamble_, pilot_grid_, constellation_;
+    DevBuf<int> pilot_symbols_, data_symbols_, used_natural_, p0_, p1_;
+    DevBuf<float> w0_, w1_;
+    DevBuf<double2> twiddle_;
+};
+
+// Gray QAM constellation indexed by the symbol's bits MSB first, unit average
+// power (configure_modulation, uhd_ldpc_ofdm_link.cpp:2362-2414).
+std::vector<float2> qam_constellation(int bits_per_symbol);
+
+}  // namespace sicc

@@ -296,6 +296,92 @@ class Decoder:
         return s
 
 
+class OfdmConfig(C.Structure):
+    _fields_ = [
+        ("nfft", C.c_int), ("cp", C.c_int), ("preamble_len", C.c_int), ("pre_half_len", C.c_int),
+        ("num_symbols", C.c_int), ("active_sc", C.c_int), ("bits_per_symbol", C.c_int), ("code_n", C.c_int),
+        ("blocks_per_frame", C.c_int), ("max_rx_search_samples", C.c_int), ("refine_radius", C.c_int),
+        ("use_pilot_channel_est", C.c_int), ("sample_rate", C.c_float), ("sync_threshold", C.c_float),
+        ("demod_noise_var", C.c_float),
+    ]
+
+
+SYNC_DTYPE = np.dtype([("start", np.int32), ("metric", np.float32), ("cfo_hz", np.float32),
+                       ("coarse_metric", np.float32), ("cfo_hz_d", np.float64)])
+
+
+class OfdmReceiver:
+    """Owns one OFDM receiver handle (sicc_ofdm_* C ABI)."""
+
+    def __init__(self, phy, code_n: int, blocks_per_frame: int, max_search: int = 0, noise_var: float = 1e-3,
+                 threshold: float = 0.45, refine_radius: int = -1, use_pilots: bool = True):
+        self.lib = load()
+        lib = self.lib
+        lib.sicc_ofdm_create.argtypes = [C.POINTER(OfdmConfig), f32p, i32p, C.c_int, i32p, f32p,
+                                         C.POINTER(C.c_ulonglong)]
+        lib.sicc_ofdm_destroy.argtypes = [C.c_ulonglong]
+        lib.sicc_ofdm_llr_per_frame.argtypes = [C.c_ulonglong]
+        lib.sicc_ofdm_receive_host.argtypes = [C.c_ulonglong, f32p, C.c_int, C.c_int, C.c_void_p, f32p]
+        lib.sicc_ofdm_receive_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p]
+        lib.sicc_ofdm_sync_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        lib.sicc_ofdm_demod_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]
+        self.cfg = OfdmConfig(nfft=phy.nfft, cp=phy.cp, preamble_len=phy.pre_len, pre_half_len=phy.pre_half_len,
+                              num_symbols=phy.num_symbols, active_sc=phy.active_sc,
+                              bits_per_symbol=phy.bits_per_symbol, code_n=code_n, blocks_per_frame=blocks_per_frame,
+                              max_rx_search_samples=max_search, refine_radius=refine_radius,
+                              use_pilot_channel_est=int(use_pilots), sample_rate=phy.fs, sync_threshold=threshold,
+                              demod_noise_var=noise_var)
+        pre = np.ascontiguousarray(np.asarray(phy.preamble, np.complex64).view(np.float32))
+        used = np.ascontiguousarray(phy.used_shifted, np.int32)
+        ps = np.ascontiguousarray(phy.pilot_symbols, np.int32)
+        pg = np.ascontiguousarray(np.asarray(phy.pilot_grid, np.complex64).reshape(-1).view(np.float32))
+        h = C.c_ulonglong(0)
+        check(lib.sicc_ofdm_create(C.byref(self.cfg), ptr(pre, f32p), ptr(used, i32p), len(ps), ptr(ps, i32p),
+                                   ptr(pg, f32p), C.byref(h)))
+        self.handle = h.value
+        self.llr_per_frame = lib.sicc_ofdm_llr_per_frame(self.handle)
+
+    def close(self):
+        if self.handle:
+            check(self.lib.sicc_ofdm_destroy(self.handle))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def receive_host(self, rx: np.ndarray):
+        rx = np.ascontiguousarray(np.asarray(rx, np.complex64))
+        frames, span = rx.shape
+        sync = np.zeros(frames, SYNC_DTYPE)
+        llr = np.zeros((frames, self.llr_per_frame), np.float32)
+        check(self.lib.sicc_ofdm_receive_host(self.handle, rx.view(np.float32).ctypes.data_as(f32p), frames, span,
+                                              sync.ctypes.data_as(C.c_void_p), ptr(llr, f32p)))
+        return sync, llr
+
+    def receive_device(self, d_rx: int, frames: int, span: int, d_sync: int, d_llr: int, stream=None):
+        check(self.lib.sicc_ofdm_receive_device(self.handle, C.c_void_p(d_rx), frames, span, C.c_void_p(d_sync),
+                                                C.c_void_p(d_llr), stream))
+
+
+def find_frame(signal: np.ndarray, preamble: np.ndarray, half_len: int, threshold: float, fs: float):
+    """cuda_osd_find_frame on the GPU -> (start_1based, peak, cfo_hz)."""
+    lib = load()
+    sr = np.ascontiguousarray(signal.real, np.float64)
+    si = np.ascontiguousarray(signal.imag, np.float64)
+    pr = np.ascontiguousarray(preamble.real, np.float64)
+    pi = np.ascontiguousarray(preamble.imag, np.float64)
+    s, pk, cf = C.c_int(), C.c_double(), C.c_double()
+    check(lib.cuda_osd_find_frame(ptr(sr, f64p), ptr(si, f64p), len(sr), ptr(pr, f64p), ptr(pi, f64p), len(pr),
+                                  half_len, C.c_double(threshold), C.c_double(fs), C.byref(s), C.byref(pk),
+                                  C.byref(cf)))
+    return s.value, pk.value, cf.value
+
+
 def qc_code(n: int, mb: int, col_weight: int = 3, seed: int = 1):
     """CSR of the seeded QC-LDPC code built by the library."""
     lib = load()
