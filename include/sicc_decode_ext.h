@@ -69,6 +69,17 @@ int sicc_set_kernel_timing(unsigned long long handle, int enable);
 int sicc_get_kernel_timing(unsigned long long handle, double* bp_ms, double* osd_ms, int* bp_launches,
                            int* osd_launches);
 
+/* BP launch shape of a decoder: wavefronts per codeword (1, 2, 4, 8, 16; 0 =
+ * default), codewords per workgroup for single-wave teams (1, 2, 4) and the
+ * persistent grid size in wavefronts (0 = 32 per CU).  The autotuner times
+ * every supported shape on `codewords` device LLR rows and keeps the fastest;
+ * it reports the choice and its average BP kernel time.  No counterpart in the
+ * reference (its decoder is a prebuilt library with fixed launch shapes). */
+int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves);
+int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves);
+int sicc_autotune(unsigned long long handle, const float* device_llrs, int codewords, int* team,
+                  int* teams_per_block, int* max_waves, double* best_ms);
+
 /* Synchronous host-buffer variant of sicc_decode_device (hard bits, soft LLRs,
  * per-frame success) for CPU callers; outputs may be null. */
 int sicc_decode_host(unsigned long long handle, const float* llrs, int codewords,

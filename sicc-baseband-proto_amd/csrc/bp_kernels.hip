@@ -10,20 +10,23 @@
 // Every float operation is the same operation, in the same order, as the CPU
 // decoder's, so outputs are bit-identical (built with -ffp-contract=off).
 //
-// Mapping: one wavefront decodes one codeword.  Lane l owns check rows
-// l, l+64, ... and variable nodes l, l+64, ...  Registers hold each lane's row
-// edge lists, its rows' previous check messages (so q = APP - r_old needs no
-// stored q) and its variables' LLR / APP / best-syndrome APP.  Per wave, LDS
-// holds only what crosses lanes: APP[n] (gathered along rows) and the new
-// check messages r[E] in column-major order (summed along columns).  The check
-// update is branch-free: padded edges point at a FLT_MAX sentinel APP, min1/min2
-// use min/max, and the first-minimum test is |q| == min1 (equal values imply
-// min2 == min1).  HBM traffic per codeword is the LLR read and the bit/flag
-// writes.  Waves are persistent and stride over the batch.
+// Mapping: a team of TEAM wavefronts decodes one codeword (TEAM = 1 for short
+// codes, up to 16 for n = 2048).  Team thread t owns check rows t, t + T, ...
+// and variable nodes t, t + T, ... (T = 64 * TEAM).  Registers hold each
+// thread's row edge lists, its rows' previous check messages (so
+// q = APP - r_old needs no stored q) and its variables' LLR / APP /
+// best-syndrome APP.  LDS holds only what crosses threads: APP[n] (gathered
+// along rows) and the new check messages r[E] in column-major order (summed
+// along columns).  The check update is branch-free: padded edges point at a
+// FLT_MAX sentinel APP, min1/min2 use min/max, and the first-minimum test is
+// |q| == min1 (equal values imply min2 == min1).  HBM traffic per codeword is
+// the LLR read and the bit/flag writes.  Teams are persistent and stride over
+// the batch; a single-wave team synchronizes with wave barriers only.
 #include "kernels.hpp"
 
 #include <cfloat>
 #include <stdexcept>
+#include <string>
 
 namespace sicc {
 
@@ -36,23 +39,35 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <int TEAM>
+__device__ __forceinline__ void team_sync()
+{
+    if (TEAM == 1)
+        wave_lds_sync();
+    else
+        __syncthreads();
+}
+
 __global__ void zero_i32_kernel(int* p, int count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0;
 }
 
-constexpr int kWavesPerBlock = 4;
-
-template <int MPL, int NPL, int MAXDC, bool DAMP>
-__global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io, int e_pad)
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
+                                                                              int e_pad)
 {
     extern __shared__ float lds[];
+    constexpr int T = 64 * TEAM;     // threads per team
+    constexpr int np = T * NPL;      // padded variable count
+    const int tid = threadIdx.x % T;
+    const int team = threadIdx.x / T;
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    constexpr int np = 64 * NPL;
-    // per wave: APP[np] + one sentinel slot (FLT_MAX) for padded edges, then r[e_pad]
-    float* app = lds + wv * (np + 4 + e_pad);
+    // per team: [8 reduction words] APP[np] + sentinel pad (4) + r[e_pad]
+    float* tbase = lds + team * (8 + np + 4 + e_pad);
+    int* red = reinterpret_cast<int*>(tbase);  // [0..1] syndrome counters, [2] OSD slot, [3] min|APP| bits
+    float* app = tbase + 8;
     float* rcol = app + np + 4;
 
     const int n = code.n;
@@ -63,7 +78,7 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
     int em[MPL][MAXDC];
 #pragma unroll
     for (int s = 0; s < MPL; ++s) {
-        const int r = lane + 64 * s;
+        const int r = tid + T * s;
 #pragma unroll
         for (int j = 0; j < MAXDC; ++j) {
             const int e = (r < m && j < code.max_dc) ? code.row_meta[r * code.max_dc + j] : -1;
@@ -73,23 +88,27 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
     int cm[NPL];
 #pragma unroll
     for (int t = 0; t < NPL; ++t) {
-        const int v = lane + 64 * t;
+        const int v = tid + T * t;
         cm[t] = v < n ? code.col_meta[v] : 0;
     }
-    if (lane == 0) app[np] = FLT_MAX;
+    if (tid == 0) {
+        app[np] = FLT_MAX;
+        red[0] = red[1] = 0;
+    }
 
-    const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
-    const int nw = gridDim.x * (blockDim.x >> 6);
+    const int teams_per_block = blockDim.x / T;
+    const int gteam = blockIdx.x * teams_per_block + team;
+    const int nteams = gridDim.x * teams_per_block;
     const float alpha = p.alpha;
     const float off = p.offset;
     const float damp = p.damping;
 
-    for (int cw = gw; cw < io.num_cw; cw += nw) {
+    for (int cw = gteam; cw < io.num_cw; cw += nteams) {
         const float* L = io.llr + static_cast<size_t>(cw) * n;
         float lr[NPL], appr[NPL], bestr[NPL];
 #pragma unroll
         for (int t = 0; t < NPL; ++t) {
-            const int v = lane + 64 * t;
+            const int v = tid + T * t;
             lr[t] = v < n ? L[v] : 0.0f;
             appr[t] = lr[t];
             bestr[t] = lr[t];
@@ -104,7 +123,7 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
                 rold[s][j] = 0.0f;
                 qold[s][j] = 0.0f;
             }
-        wave_lds_sync();
+        team_sync<TEAM>();
 
         int best_w = 0;
         bool ok = false;
@@ -151,6 +170,14 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
                     rcol[em[s][j] & 0xffff] = r;
                 }
             }
+            if (TEAM > 1) {
+                // team-wide syndrome weight through an LDS counter, double-buffered by iteration
+                int* ctr = red + (it & 1);
+                if (lane == 0 && synw) atomicAdd(ctr, synw);
+                __syncthreads();
+                synw = *ctr;
+                if (tid == 0) red[(it + 1) & 1] = 0;
+            }
             if (it > 0) {
                 if (synw < best_w || synw == 0) {
                     best_w = synw;
@@ -165,11 +192,11 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
                 best_w = synw;
             }
             if (it == p.max_iter) break;
-            wave_lds_sync();
+            team_sync<TEAM>();
             // ---- variable-node update: APP = LLR + sum r, increasing row order ----
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
-                const int v = lane + 64 * t;
+                const int v = tid + T * t;
                 if (v < n) {
                     float a = lr[t];
                     const int st = cm[t] >> 8;
@@ -179,8 +206,13 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
                     app[v] = a;
                 }
             }
-            wave_lds_sync();
+            team_sync<TEAM>();
             ++it;
+        }
+        if (TEAM > 1) {
+            // the counter the last check pass used must read zero for the next codeword
+            __syncthreads();
+            if (tid == 0) red[0] = red[1] = 0;
         }
 
         // ---- outputs ----
@@ -189,14 +221,21 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
             float mn = FLT_MAX;
 #pragma unroll
             for (int t = 0; t < NPL; ++t)
-                if (lane + 64 * t < n) mn = fminf(mn, fabsf(appr[t]));
+                if (tid + T * t < n) mn = fminf(mn, fabsf(appr[t]));
             for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
+            if (TEAM > 1) {
+                if (tid == 0) red[3] = __float_as_int(FLT_MAX);
+                __syncthreads();
+                if (lane == 0) atomicMin(red + 3, __float_as_int(mn));  // non-negative floats order as ints
+                __syncthreads();
+                mn = __int_as_float(red[3]);
+            }
             accept = mn >= p.min_abs_accept;
         }
         if (io.code_bits || io.app) {
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
-                const int v = lane + 64 * t;
+                const int v = tid + T * t;
                 if (v < n) {
                     if (io.code_bits) io.code_bits[static_cast<size_t>(cw) * n + v] = appr[t] < 0.0f ? 1 : 0;
                     if (io.app) io.app[static_cast<size_t>(cw) * n + v] = appr[t];
@@ -205,10 +244,10 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
         }
         if (io.info_bits) {
             const int k = code.k;
-            for (int i = lane; i < k; i += 64)
+            for (int i = tid; i < k; i += T)
                 io.info_bits[static_cast<size_t>(cw) * k + i] = app[code.info_cols[i]] < 0.0f ? 1 : 0;
         }
-        if (lane == 0) {
+        if (tid == 0) {
             if (io.success) io.success[cw] = accept ? 1 : 0;
             if (io.iterations) io.iterations[cw] = it;
             if (io.used_osd) io.used_osd[cw] = (!accept && io.osd_llr) ? 1 : 0;
@@ -216,43 +255,74 @@ __global__ __launch_bounds__(256) void bp_flood_kernel(BpCodeDev code, BpParams 
         }
         if (!accept && io.osd_llr) {
             int slot = 0;
-            if (lane == 0) slot = atomicAdd(io.osd_count, 1);
-            slot = __shfl(slot, 0);
+            if (TEAM == 1) {
+                if (lane == 0) slot = atomicAdd(io.osd_count, 1);
+                slot = __shfl(slot, 0);
+            } else {
+                if (tid == 0) red[2] = atomicAdd(io.osd_count, 1);
+                __syncthreads();
+                slot = red[2];
+            }
             float* dst = io.osd_llr + static_cast<size_t>(slot) * n;
             // a converged-but-rejected frame hands over its final APP (== best)
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
-                const int v = lane + 64 * t;
+                const int v = tid + T * t;
                 if (v < n) dst[v] = bestr[t];
             }
-            if (lane == 0) io.osd_index[slot] = cw;
+            if (tid == 0) io.osd_index[slot] = cw;
         }
-        wave_lds_sync();
+        team_sync<TEAM>();
     }
 }
 
-template <int MPL, int NPL, int MAXDC>
-void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, hipStream_t s)
+template <int MPL, int NPL, int MAXDC, int TEAM>
+void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, int tpb_req,
+                    hipStream_t s)
 {
     const int e_pad = (code.E + 1 + 63) & ~63;  // one spare slot for padded edges
     const bool damp = p.damping != 0.0f;
-    const size_t wave_bytes = sizeof(float) * static_cast<size_t>(64 * NPL + 4 + e_pad);
-    // At most 64 KiB of LDS per workgroup; fewer waves per group for big codes.
-    int wpb = static_cast<int>((64 * 1024) / wave_bytes);
-    if (wpb > kWavesPerBlock) wpb = kWavesPerBlock;
-    if (wpb < 1) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
-    const size_t lds = wave_bytes * wpb;
-    int waves = io.num_cw < max_waves ? io.num_cw : max_waves;
-    if (waves < 1) waves = 1;
-    const int blocks = (waves + wpb - 1) / wpb;
+    constexpr int T = 64 * TEAM;
+    const size_t team_bytes = sizeof(float) * static_cast<size_t>(8 + T * NPL + 4 + e_pad);
+    // Up to 4 single-wave teams per group; a multi-wave team synchronizes with
+    // group barriers, so it must be alone in its group.  Within 64 KiB of LDS.
+    int tpb = TEAM == 1 ? (tpb_req >= 4 ? 4 : (tpb_req >= 2 ? 2 : 1)) : 1;
+    while (tpb > 1 && team_bytes * tpb > 64 * 1024) --tpb;
+    if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
+    const size_t lds = team_bytes * tpb;
+    int teams = io.num_cw < max_waves / TEAM ? io.num_cw : max_waves / TEAM;
+    if (teams < 1) teams = 1;
+    const int blocks = (teams + tpb - 1) / tpb;
     if (damp)
-        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, true>), dim3(blocks), dim3(64 * wpb), lds, s, code, p,
-                           io, e_pad);
+        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM>), dim3(blocks), dim3(T * tpb), lds, s, code,
+                           p, io, e_pad);
     else
-        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, false>), dim3(blocks), dim3(64 * wpb), lds, s, code, p,
-                           io, e_pad);
+        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>), dim3(blocks), dim3(T * tpb), lds, s, code,
+                           p, io, e_pad);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("bp_flood_kernel launch: ") + hipGetErrorString(e));
+}
+
+template <int MPL, int NPL, int MAXDC>
+void launch_team(int team, int tpb, const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves,
+                 hipStream_t s)
+{
+    switch (team) {
+        case 1: return launch_variant<MPL, NPL, MAXDC, 1>(code, p, io, max_waves, tpb, s);
+        case 2: return launch_variant<MPL, NPL, MAXDC, 2>(code, p, io, max_waves, tpb, s);
+        case 4: return launch_variant<MPL, NPL, MAXDC, 4>(code, p, io, max_waves, tpb, s);
+        case 8: return launch_variant<MPL, NPL, MAXDC, 8>(code, p, io, max_waves, tpb, s);
+        case 16: return launch_variant<MPL, NPL, MAXDC, 16>(code, p, io, max_waves, tpb, s);
+        default: throw std::invalid_argument("BP team size must be 1, 2, 4, 8 or 16 waves");
+    }
+}
+
+// Rows and variables per thread for a team of `team` waves.
+void per_thread(const BpCodeDev& c, int team, int* mpl, int* npl)
+{
+    const int T = 64 * team;
+    *mpl = (c.m + T - 1) / T;
+    *npl = (c.n + T - 1) / T;
 }
 
 }  // namespace
@@ -263,37 +333,51 @@ void launch_zero_i32(int* p, int count, hipStream_t s)
     hipLaunchKernelGGL(zero_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, s, p, count);
 }
 
-bool bp_shape_supported(const BpCodeDev& c)
+bool bp_team_supported(const BpCodeDev& c, int team)
 {
-    if (c.n <= 0 || c.m <= 0 || c.E <= 0 || c.E > 0xffff || c.n > 0x7fff) return false;
-    const int mpl = (c.m + 63) / 64;
-    const int npl = (c.n + 63) / 64;
-    if (c.max_dc > 16) return false;
-    if (c.max_dc > 8) return mpl <= 4 && npl <= 8;
-    return mpl <= 8 && npl <= 16;
+    if (c.n <= 0 || c.m <= 0 || c.E <= 0 || c.E > 0xfffe || c.n > 0x7fff || c.max_dc > 16) return false;
+    if (team != 1 && team != 2 && team != 4 && team != 8 && team != 16) return false;
+    int mpl = 0, npl = 0;
+    per_thread(c, team, &mpl, &npl);
+    if (mpl > 2 || npl > 4) return false;
+    const bool small = mpl <= 1 && npl <= 2;
+    const int e_pad = (c.E + 1 + 63) & ~63;
+    const size_t np = static_cast<size_t>(64) * team * (small ? 2 : 4);
+    return sizeof(float) * (8 + np + 4 + static_cast<size_t>(e_pad)) <= 64 * 1024;
 }
 
-void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, hipStream_t s)
+int bp_default_team(const BpCodeDev& c)
 {
-    if (!bp_shape_supported(code)) throw std::invalid_argument("code shape not supported by the BP kernel");
+    for (int team : {1, 2, 4, 8, 16}) {
+        int mpl = 0, npl = 0;
+        per_thread(c, team, &mpl, &npl);
+        if (mpl <= 1 && npl <= 2 && bp_team_supported(c, team)) return team;
+    }
+    for (int team : {1, 2, 4, 8, 16})
+        if (bp_team_supported(c, team)) return team;
+    return 0;
+}
+
+bool bp_shape_supported(const BpCodeDev& c) { return bp_default_team(c) != 0; }
+
+void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, hipStream_t s,
+                     const BpTune& tune)
+{
+    const int team = tune.team > 0 ? tune.team : bp_default_team(code);
+    if (!bp_team_supported(code, team)) throw std::invalid_argument("code shape not supported by the BP kernel");
     if (io.num_cw <= 0) return;
     if (p.max_iter < 1) throw std::invalid_argument("max_iter must be >= 1");
-    const int mpl = (code.m + 63) / 64;
-    const int npl = (code.n + 63) / 64;
-    const int dc = code.max_dc <= 8 ? 8 : 16;
-    // Round the shape up to an instantiated variant.
-    const int mp = mpl <= 1 ? 1 : mpl <= 2 ? 2 : mpl <= 4 ? 4 : 8;
-    const int nq = npl <= 2 ? 2 : npl <= 4 ? 4 : npl <= 8 ? 8 : 16;
-    if (dc == 8) {
-        if (mp == 1 && nq == 2) return launch_variant<1, 2, 8>(code, p, io, max_waves, s);
-        if (mp <= 2 && nq <= 4) return launch_variant<2, 4, 8>(code, p, io, max_waves, s);
-        if (mp <= 4 && nq <= 8) return launch_variant<4, 8, 8>(code, p, io, max_waves, s);
-        return launch_variant<8, 16, 8>(code, p, io, max_waves, s);
+    if (tune.max_waves > 0) max_waves = tune.max_waves;
+    const int tpb = tune.teams_per_block;
+    int mpl = 0, npl = 0;
+    per_thread(code, team, &mpl, &npl);
+    const bool small = mpl <= 1 && npl <= 2;
+    if (code.max_dc <= 8) {
+        if (small) return launch_team<1, 2, 8>(team, tpb, code, p, io, max_waves, s);
+        return launch_team<2, 4, 8>(team, tpb, code, p, io, max_waves, s);
     }
-    if (mp == 1 && nq == 2) return launch_variant<1, 2, 16>(code, p, io, max_waves, s);
-    if (mp <= 2 && nq <= 4) return launch_variant<2, 4, 16>(code, p, io, max_waves, s);
-    if (mp <= 4 && nq <= 8) return launch_variant<4, 8, 16>(code, p, io, max_waves, s);
-    throw std::invalid_argument("code shape not supported by the BP kernel");
+    if (small) return launch_team<1, 2, 16>(team, tpb, code, p, io, max_waves, s);
+    return launch_team<2, 4, 16>(team, tpb, code, p, io, max_waves, s);
 }
 
 }  // namespace sicc

@@ -825,6 +825,47 @@ int sicc_decode_device(unsigned long long handle, const float* device_llrs, int 
     });
 }
 
+int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves)
+{
+    return guarded([&] {
+        auto d = lookup(handle, "set_bp_tune");
+        BpTune t;
+        t.team = team;
+        t.teams_per_block = teams_per_block;
+        t.max_waves = max_waves;
+        d->eng->set_tune(t);
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves)
+{
+    return guarded([&] {
+        auto d = lookup(handle, "get_bp_tune");
+        const BpTune& t = d->eng->tune();
+        if (team) *team = t.team;
+        if (teams_per_block) *teams_per_block = t.teams_per_block;
+        if (max_waves) *max_waves = t.max_waves > 0 ? t.max_waves : d->eng->max_waves();
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_autotune(unsigned long long handle, const float* device_llrs, int codewords, int* team, int* teams_per_block,
+                  int* max_waves, double* best_ms)
+{
+    return guarded([&] {
+        auto d = lookup(handle, "autotune");
+        const double ms = d->eng->autotune(device_llrs, codewords, d->bp, d->stream);
+        SICC_HIP(hipStreamSynchronize(d->stream));
+        const BpTune& t = d->eng->tune();
+        if (team) *team = t.team;
+        if (teams_per_block) *teams_per_block = t.teams_per_block;
+        if (max_waves) *max_waves = t.max_waves;
+        if (best_ms) *best_ms = ms;
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
 int sicc_set_kernel_timing(unsigned long long handle, int enable)
 {
     return guarded([&] {

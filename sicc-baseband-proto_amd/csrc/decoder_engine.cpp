@@ -54,6 +54,7 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
     bpdev_.col_meta = col_meta_.get();
     bpdev_.info_cols = info_cols_.get();
     if (!bp_shape_supported(bpdev_)) throw std::invalid_argument("code shape not supported by the BP kernel");
+    tune_.team = bp_default_team(bpdev_);
 
     osd_ok_ = osd_shape_supported(n, code_.k);
     if (osd_ok_) {
@@ -112,7 +113,7 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         io.osd_count = osd_count_.get();
     }
     timed_begin(0, s);
-    launch_bp_flood(bpdev_, bp, io, max_waves_, s);
+    launch_bp_flood(bpdev_, bp, io, max_waves_, s, tune_);
     timed_end(s);
     if (use_osd) {
         OsdParams op = osd;
@@ -144,6 +145,70 @@ void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, cons
     timed_begin(1, s);
     launch_osd(osddev_, osd, oio, num_cw, s);
     timed_end(s);
+}
+
+void Engine::set_tune(const BpTune& t)
+{
+    const int team = t.team > 0 ? t.team : bp_default_team(bpdev_);
+    if (!bp_team_supported(bpdev_, team)) throw std::invalid_argument("BP launch shape not supported for this code");
+    if (t.teams_per_block != 1 && t.teams_per_block != 2 && t.teams_per_block != 4)
+        throw std::invalid_argument("teams_per_block must be 1, 2 or 4");
+    if (t.max_waves < 0) throw std::invalid_argument("max_waves must be >= 0");
+    tune_ = t;
+    tune_.team = team;
+}
+
+double Engine::autotune(const float* d_llr, int num_cw, const BpParams& bp, hipStream_t s)
+{
+    if (!d_llr || num_cw <= 0) throw std::invalid_argument("autotune needs a sample batch");
+    DevBuf<uint8_t> bits(static_cast<size_t>(num_cw) * code_.k);
+    DevBuf<uint8_t> ok(num_cw);
+    BpIo io;
+    io.llr = d_llr;
+    io.num_cw = num_cw;
+    io.info_bits = bits.get();
+    io.success = ok.get();
+    hipEvent_t e0, e1;
+    SICC_HIP(hipEventCreate(&e0));
+    SICC_HIP(hipEventCreate(&e1));
+    int dev = 0, cus = 256;
+    SICC_HIP(hipGetDevice(&dev));
+    SICC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BpTune best = tune_;
+    double best_ms = 1e30;
+    try {
+        for (int team : {1, 2, 4, 8, 16}) {
+            if (!bp_team_supported(bpdev_, team)) continue;
+            for (int tpb : {1, 2, 4}) {
+                if (team > 1 && tpb > 1) continue;
+                for (int wpc : {8, 16, 32}) {
+                    BpTune t;
+                    t.team = team;
+                    t.teams_per_block = tpb;
+                    t.max_waves = cus * wpc;
+                    launch_bp_flood(bpdev_, bp, io, max_waves_, s, t);  // warm-up
+                    SICC_HIP(hipEventRecord(e0, s));
+                    for (int r = 0; r < 3; ++r) launch_bp_flood(bpdev_, bp, io, max_waves_, s, t);
+                    SICC_HIP(hipEventRecord(e1, s));
+                    SICC_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    SICC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms / 3.0 < best_ms) {
+                        best_ms = ms / 3.0;
+                        best = t;
+                    }
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_ = best;
+    return best_ms;
 }
 
 void Engine::set_timing(bool on)

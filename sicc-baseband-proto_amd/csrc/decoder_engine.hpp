@@ -40,6 +40,13 @@ public:
 
     int max_waves() const { return max_waves_; }
 
+    // BP launch shape.  autotune() times every supported shape (wavefronts per
+    // codeword, codewords per workgroup, persistent grid size) on a sample batch
+    // already in HBM and keeps the fastest; it writes only internal scratch.
+    const BpTune& tune() const { return tune_; }
+    void set_tune(const BpTune& t);
+    double autotune(const float* d_llr, int num_cw, const BpParams& bp, hipStream_t s);
+
     // Kernel timing with HIP events recorded on the launch stream around each
     // BP and OSD launch.  collect_timing() waits for the recorded launches and
     // returns their summed durations since the last reset.
@@ -55,6 +62,7 @@ private:
     void timed_begin(int kind, hipStream_t s);
     void timed_end(hipStream_t s);
     void drain_timing(bool all);
+    BpTune tune_;
     bool timing_ = false;
     std::vector<TimedPair> tpairs_;
     int tnext_ = 0;

@@ -81,8 +81,19 @@ struct OsdIo {
 // std::invalid_argument when a code does not fit an instantiated variant.
 void launch_zero_i32(int* p, int count, hipStream_t s);
 bool bp_shape_supported(const BpCodeDev& c);
+// Launch shape of the BP kernel (the autotuner's search space):
+//   team            wavefronts cooperating on one codeword (1, 2, 4, 8, 16); 0 = default
+//   teams_per_block codewords in flight per workgroup for single-wave teams (1, 2, 4)
+//   max_waves       persistent-grid size in wavefronts (0 = 32 per CU)
+struct BpTune {
+    int team = 0;
+    int teams_per_block = 4;
+    int max_waves = 0;
+};
+bool bp_team_supported(const BpCodeDev& c, int team);
+int bp_default_team(const BpCodeDev& c);
 void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves,
-                     hipStream_t s);
+                     hipStream_t s, const BpTune& tune = BpTune());
 bool osd_shape_supported(int n, int k);
 void launch_osd(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames,
                 hipStream_t s);

@@ -237,6 +237,23 @@ class Decoder:
         check(self.lib.sicc_decode_device(self.handle, C.c_void_p(d_llr), cw, d_info, d_code, d_app, d_ok, d_used,
                                           d_iters, d_dist, stream))
 
+    def set_tune(self, team: int = 0, teams_per_block: int = 4, max_waves: int = 0):
+        self.lib.sicc_set_bp_tune.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
+        check(self.lib.sicc_set_bp_tune(self.handle, team, teams_per_block, max_waves))
+
+    def get_tune(self):
+        t, b, w = C.c_int(), C.c_int(), C.c_int()
+        self.lib.sicc_get_bp_tune.argtypes = [C.c_ulonglong, i32p, i32p, i32p]
+        check(self.lib.sicc_get_bp_tune(self.handle, C.byref(t), C.byref(b), C.byref(w)))
+        return t.value, b.value, w.value
+
+    def autotune(self, d_llr: int, cw: int):
+        t, b, w, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self.lib.sicc_autotune.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, i32p, i32p, i32p, f64p]
+        check(self.lib.sicc_autotune(self.handle, C.c_void_p(d_llr), cw, C.byref(t), C.byref(b), C.byref(w),
+                                     C.byref(ms)))
+        return {"team": t.value, "teams_per_block": b.value, "max_waves": w.value, "bp_ms": ms.value}
+
     def set_timing(self, on: bool):
         check(self.lib.sicc_set_kernel_timing(self.handle, int(on)))
 

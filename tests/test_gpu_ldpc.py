@@ -140,6 +140,44 @@ def test_bp_csr_decoder_other_codes():
             assert np.array_equal(bits[i], o["hard"])
 
 
+@pytest.mark.parametrize("which", ["ccsds128", "ccsds256", "qc1024", "qc2048_r34", "qc512_r23"])
+def test_every_launch_shape_bit_exact(which):
+    # every (team, codewords-per-group) shape the autotuner may pick decodes
+    # bit-identically to the oracle, including the multi-wave team kernels
+    import os
+
+    import torch
+
+    if which.startswith("ccsds"):
+        n = int(which[5:])
+        path = os.path.join(sa.DATA_DIR, f"CCSDS_ldpc_n{n}_k{n // 2}.alist")
+        n, m, rp, ci = oracle.alist_parse(path)
+    else:
+        n = int(which[2:6].rstrip("_r"))
+        mb = {"": 8, "r34": 4, "r23": 5}[which.split("_")[1] if "_" in which else ""]
+        m, rp, ci = sa.qc_code(n, mb, 3, seed=n + mb)
+    print(f"[case] launch shapes {which} n={n} m={m}", flush=True)
+    code = oracle.Code(n, m, rp, ci, front_info=True)
+    _, _, llr = _frames(code, 96, 1.5 if n <= 256 else 2.0, 3 * n + m)
+    ref = [code.nms(llr[i], max_iter=25, alpha=0.75) for i in range(llr.shape[0])]
+    d_llr = torch.from_numpy(llr).cuda()
+    with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=25, alpha=0.75) as dec:
+        for team in (1, 2, 4, 8, 16):
+            for tpb in ((1, 2, 4) if team == 1 else (1,)):
+                try:
+                    dec.set_tune(team, tpb, 0)
+                except sa.SiccError:
+                    continue
+                bits, ok, _ = dec.bp_csr_batch(llr)
+                for i in range(llr.shape[0]):
+                    assert ok[i] == ref[i]["ok"], (team, tpb, i)
+                    assert np.array_equal(bits[i], ref[i]["hard"]), (team, tpb, i)
+        best = dec.autotune(d_llr.data_ptr(), llr.shape[0])
+        assert best["team"] in (1, 2, 4, 8, 16) and best["bp_ms"] > 0
+        bits, ok, _ = dec.bp_csr_batch(llr)
+        assert all(np.array_equal(bits[i], ref[i]["hard"]) for i in range(llr.shape[0]))
+
+
 def test_error_paths():
     lib = sa.load()
     import ctypes as C
