@@ -71,14 +71,39 @@ int sicc_get_kernel_timing(unsigned long long handle, double* bp_ms, double* osd
 
 /* BP launch shape of a decoder: wavefronts per codeword (1, 2, 4, 8, 16; 0 =
  * default), codewords per workgroup for single-wave teams (1, 2, 4) and the
- * persistent grid size in wavefronts (0 = 32 per CU).  The autotuner times
+ * persistent grid size in wavefronts (0 = 32 per CU), and work stealing
+ * (steal_chunk > 0: teams take that many codewords at a time from a shared
+ * cursor; 0: static striding).  The autotuner times
  * every supported shape on `codewords` device LLR rows and keeps the fastest;
  * it reports the choice and its average BP kernel time.  No counterpart in the
  * reference (its decoder is a prebuilt library with fixed launch shapes). */
-int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves);
-int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves);
+int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves,
+                     int steal_chunk);
+int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves,
+                     int* steal_chunk);
 int sicc_autotune(unsigned long long handle, const float* device_llrs, int codewords, int* team,
-                  int* teams_per_block, int* max_waves, double* best_ms);
+                  int* teams_per_block, int* max_waves, int* steal_chunk, double* best_ms);
+
+/* Mixed-code batches.  A multi-decoder groups existing decoder handles (one per
+ * code; it keeps them alive).  One call decodes any number of groups, each a
+ * contiguous device batch of one member's code, with every group's persistent
+ * BP kernel (and OSD for stream members) launched concurrently on forked
+ * streams; with use_graph the whole fork/join sequence is captured once into a
+ * hipGraph and replayed while the group descriptors stay the same.  Outputs may
+ * be null.  No counterpart in the reference (it decodes one code per handle). */
+typedef struct sicc_group_t {
+    int member;                   /* index into the decoders given at creation */
+    int count;                    /* codewords in this group */
+    const float* d_llr;           /* [count][n] */
+    unsigned char* d_info;        /* [count][k] */
+    unsigned char* d_bp_ok;       /* [count] */
+    int* d_iterations;            /* [count] */
+    unsigned char* d_used_osd;    /* [count] */
+} sicc_group_t;
+int sicc_multi_create(const unsigned long long* decoders, int count, unsigned long long* out_handle);
+int sicc_multi_destroy(unsigned long long handle);
+int sicc_multi_decode_device(unsigned long long handle, const sicc_group_t* groups, int n_groups,
+                             int use_graph, void* hip_stream);
 
 /* Synchronous host-buffer variant of sicc_decode_device (hard bits, soft LLRs,
  * per-frame success) for CPU callers; outputs may be null. */

@@ -103,7 +103,25 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
     const float off = p.offset;
     const float damp = p.damping;
 
-    for (int cw = gteam; cw < io.num_cw; cw += nteams) {
+    // Codeword assignment: static striding, or chunks taken from a global
+    // counter (work stealing) so teams that finish early take more work.
+    const bool steal = io.work_ctr != nullptr;
+    const int chunk = io.chunk > 0 ? io.chunk : 1;
+    int cw = gteam, lim = 0;
+    if (steal) {
+        int b = 0;
+        if (TEAM == 1) {
+            if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
+            b = __shfl(b, 0);
+        } else {
+            if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
+            __syncthreads();
+            b = red[4];
+        }
+        cw = b;
+        lim = min(b + chunk, io.num_cw);
+    }
+    while (cw < io.num_cw) {
         const float* L = io.llr + static_cast<size_t>(cw) * n;
         float lr[NPL], appr[NPL], bestr[NPL];
 #pragma unroll
@@ -273,6 +291,22 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
             if (tid == 0) io.osd_index[slot] = cw;
         }
         team_sync<TEAM>();
+        if (!steal) {
+            cw += nteams;
+        } else if (++cw >= lim) {
+            int b = 0;
+            if (TEAM == 1) {
+                if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
+                b = __shfl(b, 0);
+            } else {
+                if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
+                __syncthreads();
+                b = red[4];
+                __syncthreads();
+            }
+            cw = b;
+            lim = min(b + chunk, io.num_cw);
+        }
     }
 }
 

@@ -378,6 +378,67 @@ void ensure_scratch(Decoder& d, int count, bool app)
     if (app && d.s_app.size() < need * n) d.s_app.alloc(need * n);
 }
 
+// ---- mixed-code batches ----
+
+struct MultiDec {
+    int device = 0;
+    std::vector<std::shared_ptr<Decoder>> members;
+    hipStream_t main = nullptr;
+    std::vector<hipStream_t> lanes;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<sicc_group_t> key;
+    std::vector<const void*> key_tunes;
+
+    ~MultiDec()
+    {
+        if (main) (void)hipStreamSynchronize(main);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto s : lanes) (void)hipStreamDestroy(s);
+        for (auto e : joins) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (main) (void)hipStreamDestroy(main);
+    }
+};
+
+HandleTable<MultiDec>& mtable()
+{
+    static HandleTable<MultiDec> t;
+    return t;
+}
+
+bool same_groups(const std::vector<sicc_group_t>& a, const sicc_group_t* b, int n)
+{
+    if (static_cast<int>(a.size()) != n) return false;
+    for (int i = 0; i < n; ++i)
+        if (std::memcmp(&a[i], &b[i], sizeof(sicc_group_t)) != 0) return false;
+    return true;
+}
+
+// Enqueue every group on its own lane stream, forked from and joined back to `s`.
+void enqueue_groups(MultiDec& md, const sicc_group_t* groups, int n, hipStream_t s)
+{
+    SICC_HIP(hipEventRecord(md.fork, s));
+    for (int i = 0; i < n; ++i) {
+        const sicc_group_t& g = groups[i];
+        Decoder& d = *md.members[g.member];
+        hipStream_t ls = md.lanes[i % md.lanes.size()];
+        SICC_HIP(hipStreamWaitEvent(ls, md.fork, 0));
+        DecodeOut out;
+        out.info = g.d_info;
+        out.bp_ok = g.d_bp_ok;
+        out.iterations = g.d_iterations;
+        out.used_osd = g.d_used_osd;
+        d.eng->decode(g.d_llr, g.count, d.bp, d.kind == Kind::Stream ? d.osd_order : -1, d.osd, out, ls);
+        SICC_HIP(hipEventRecord(md.joins[i % md.joins.size()], ls));
+    }
+    const int used = std::min<int>(n, static_cast<int>(md.lanes.size()));
+    for (int i = 0; i < used; ++i) SICC_HIP(hipStreamWaitEvent(s, md.joins[i], 0));
+}
+
 }  // namespace
 }  // namespace sicc
 
@@ -825,7 +886,7 @@ int sicc_decode_device(unsigned long long handle, const float* device_llrs, int 
     });
 }
 
-int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves)
+int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, int max_waves, int steal_chunk)
 {
     return guarded([&] {
         auto d = lookup(handle, "set_bp_tune");
@@ -833,12 +894,13 @@ int sicc_set_bp_tune(unsigned long long handle, int team, int teams_per_block, i
         t.team = team;
         t.teams_per_block = teams_per_block;
         t.max_waves = max_waves;
+        t.steal_chunk = steal_chunk;
         d->eng->set_tune(t);
         return CUDA_OSD_STATUS_OK;
     });
 }
 
-int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves)
+int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block, int* max_waves, int* steal_chunk)
 {
     return guarded([&] {
         auto d = lookup(handle, "get_bp_tune");
@@ -846,12 +908,13 @@ int sicc_get_bp_tune(unsigned long long handle, int* team, int* teams_per_block,
         if (team) *team = t.team;
         if (teams_per_block) *teams_per_block = t.teams_per_block;
         if (max_waves) *max_waves = t.max_waves > 0 ? t.max_waves : d->eng->max_waves();
+        if (steal_chunk) *steal_chunk = t.steal_chunk;
         return CUDA_OSD_STATUS_OK;
     });
 }
 
 int sicc_autotune(unsigned long long handle, const float* device_llrs, int codewords, int* team, int* teams_per_block,
-                  int* max_waves, double* best_ms)
+                  int* max_waves, int* steal_chunk, double* best_ms)
 {
     return guarded([&] {
         auto d = lookup(handle, "autotune");
@@ -861,7 +924,98 @@ int sicc_autotune(unsigned long long handle, const float* device_llrs, int codew
         if (team) *team = t.team;
         if (teams_per_block) *teams_per_block = t.teams_per_block;
         if (max_waves) *max_waves = t.max_waves;
+        if (steal_chunk) *steal_chunk = t.steal_chunk;
         if (best_ms) *best_ms = ms;
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_multi_create(const unsigned long long* decoders, int count, unsigned long long* out_handle)
+{
+    return guarded([&] {
+        if (!out_handle || !decoders || count <= 0) throw std::invalid_argument("multi_create: no decoders");
+        *out_handle = 0;
+        auto md = std::make_unique<MultiDec>();
+        SICC_HIP(hipGetDevice(&md->device));
+        for (int i = 0; i < count; ++i) {
+            auto d = table().get(decoders[i]);
+            if (!d || d->kind == Kind::OsdOnly) throw std::invalid_argument("multi_create: members must be BP decoders");
+            if (d->device != md->device) throw std::invalid_argument("multi_create: members on another device");
+            md->members.push_back(d);
+        }
+        SICC_HIP(hipStreamCreateWithFlags(&md->main, hipStreamNonBlocking));
+        SICC_HIP(hipEventCreateWithFlags(&md->fork, hipEventDisableTiming));
+        for (int i = 0; i < 8; ++i) {
+            hipStream_t s = nullptr;
+            hipEvent_t e = nullptr;
+            SICC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            SICC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            md->lanes.push_back(s);
+            md->joins.push_back(e);
+        }
+        *out_handle = mtable().add(std::move(md));
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_multi_destroy(unsigned long long handle)
+{
+    return guarded([&] {
+        if (!mtable().remove(handle)) throw std::invalid_argument("multi_destroy: unknown handle");
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_multi_decode_device(unsigned long long handle, const sicc_group_t* groups, int n_groups, int use_graph,
+                             void* hip_stream)
+{
+    return guarded([&] {
+        auto md = mtable().get(handle);
+        if (!md) throw std::invalid_argument("multi_decode: unknown handle");
+        SICC_HIP(hipSetDevice(md->device));
+        if (!groups || n_groups <= 0) throw std::invalid_argument("multi_decode: no groups");
+        for (int i = 0; i < n_groups; ++i) {
+            const sicc_group_t& g = groups[i];
+            if (g.member < 0 || g.member >= static_cast<int>(md->members.size()) || g.count <= 0 || !g.d_llr)
+                throw std::invalid_argument("multi_decode: bad group " + std::to_string(i));
+        }
+        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : md->main;
+        if (!use_graph) {
+            enqueue_groups(*md, groups, n_groups, s);
+            return CUDA_OSD_STATUS_OK;
+        }
+        std::vector<const void*> tunes;
+        for (int i = 0; i < n_groups; ++i) {
+            const BpTune& t = md->members[groups[i].member]->eng->tune();
+            tunes.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(
+                (t.team << 24) ^ (t.teams_per_block << 20) ^ (t.max_waves << 4) ^ t.steal_chunk)));
+        }
+        if (!md->exec || !same_groups(md->key, groups, n_groups) || tunes != md->key_tunes) {
+            if (md->exec) SICC_HIP(hipGraphExecDestroy(md->exec));
+            if (md->graph) SICC_HIP(hipGraphDestroy(md->graph));
+            md->exec = nullptr;
+            md->graph = nullptr;
+            for (int i = 0; i < n_groups; ++i) {
+                Decoder& d = *md->members[groups[i].member];
+                d.eng->set_timing(false);
+                if (d.kind == Kind::Stream && d.osd_order >= 0) d.eng->reserve(groups[i].count);
+            }
+            SICC_HIP(hipStreamSynchronize(md->main));
+            SICC_HIP(hipStreamBeginCapture(md->main, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_groups(*md, groups, n_groups, md->main);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(md->main, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            SICC_HIP(hipStreamEndCapture(md->main, &md->graph));
+            SICC_HIP(hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0));
+            md->key.assign(groups, groups + n_groups);
+            md->key_tunes = tunes;
+        }
+        SICC_HIP(hipGraphLaunch(md->exec, s));
         return CUDA_OSD_STATUS_OK;
     });
 }

@@ -70,7 +70,7 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
         osddev_.gen_cols = gen_cols_.get();
         osddev_.info_cols = info_cols_.get();
     }
-    osd_count_.alloc(1);
+    osd_count_.alloc(2);
 
     int dev = 0, cus = 256;
     SICC_HIP(hipGetDevice(&dev));
@@ -105,12 +105,19 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
     io.iterations = out.iterations;
     io.used_osd = out.used_osd;
     io.distance = out.distance;
+    // counters: [0] OSD hand-off slots, [1] work-stealing cursor; cleared by a
+    // kernel (no memset node when this sequence is captured in a graph)
+    const bool steal = tune_.steal_chunk > 0;
+    if (use_osd || steal) launch_zero_i32(osd_count_.get(), 2, s);
     if (use_osd) {
         ensure_scratch(num_cw);
-        launch_zero_i32(osd_count_.get(), 1, s);
         io.osd_llr = osd_llr_.get();
         io.osd_index = osd_index_.get();
         io.osd_count = osd_count_.get();
+    }
+    if (steal) {
+        io.work_ctr = osd_count_.get() + 1;
+        io.chunk = tune_.steal_chunk;
     }
     timed_begin(0, s);
     launch_bp_flood(bpdev_, bp, io, max_waves_, s, tune_);
@@ -153,7 +160,7 @@ void Engine::set_tune(const BpTune& t)
     if (!bp_team_supported(bpdev_, team)) throw std::invalid_argument("BP launch shape not supported for this code");
     if (t.teams_per_block != 1 && t.teams_per_block != 2 && t.teams_per_block != 4)
         throw std::invalid_argument("teams_per_block must be 1, 2 or 4");
-    if (t.max_waves < 0) throw std::invalid_argument("max_waves must be >= 0");
+    if (t.max_waves < 0 || t.steal_chunk < 0) throw std::invalid_argument("max_waves and steal_chunk must be >= 0");
     tune_ = t;
     tune_.team = team;
 }
@@ -181,23 +188,34 @@ double Engine::autotune(const float* d_llr, int num_cw, const BpParams& bp, hipS
             if (!bp_team_supported(bpdev_, team)) continue;
             for (int tpb : {1, 2, 4}) {
                 if (team > 1 && tpb > 1) continue;
-                for (int wpc : {8, 16, 32}) {
-                    BpTune t;
-                    t.team = team;
-                    t.teams_per_block = tpb;
-                    t.max_waves = cus * wpc;
-                    launch_bp_flood(bpdev_, bp, io, max_waves_, s, t);  // warm-up
-                    SICC_HIP(hipEventRecord(e0, s));
-                    for (int r = 0; r < 3; ++r) launch_bp_flood(bpdev_, bp, io, max_waves_, s, t);
-                    SICC_HIP(hipEventRecord(e1, s));
-                    SICC_HIP(hipEventSynchronize(e1));
-                    float ms = 0.0f;
-                    SICC_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    if (ms / 3.0 < best_ms) {
-                        best_ms = ms / 3.0;
-                        best = t;
+                for (int wpc : {8, 16, 32})
+                    for (int sc : {0, 4}) {
+                        BpTune t;
+                        t.team = team;
+                        t.teams_per_block = tpb;
+                        t.max_waves = cus * wpc;
+                        t.steal_chunk = sc;
+                        BpIo tio = io;
+                        if (sc > 0) {
+                            tio.work_ctr = osd_count_.get() + 1;
+                            tio.chunk = sc;
+                        }
+                        auto run = [&] {
+                            if (sc > 0) launch_zero_i32(osd_count_.get() + 1, 1, s);
+                            launch_bp_flood(bpdev_, bp, tio, max_waves_, s, t);
+                        };
+                        run();  // warm-up
+                        SICC_HIP(hipEventRecord(e0, s));
+                        for (int r = 0; r < 3; ++r) run();
+                        SICC_HIP(hipEventRecord(e1, s));
+                        SICC_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        SICC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (ms / 3.0 < best_ms) {
+                            best_ms = ms / 3.0;
+                            best = t;
+                        }
                     }
-                }
             }
         }
     } catch (...) {

@@ -39,6 +39,9 @@ public:
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 
     int max_waves() const { return max_waves_; }
+    // Pre-size the OSD hand-off scratch for batches of up to num_cw codewords
+    // (decode() grows it on demand, which a graph capture cannot do).
+    void reserve(int num_cw) { ensure_scratch(num_cw); }
 
     // BP launch shape.  autotune() times every supported shape (wavefronts per
     // codeword, codewords per workgroup, persistent grid size) on a sample batch

@@ -48,6 +48,10 @@ struct BpIo {
     float* osd_llr = nullptr;     // [num_cw][n]
     int* osd_index = nullptr;     // [num_cw]
     int* osd_count = nullptr;     // one counter, zeroed before the launch
+    // Work stealing: teams take `chunk` codewords at a time from this counter
+    // (zeroed before the launch).  Null: static striding over the batch.
+    int* work_ctr = nullptr;
+    int chunk = 1;
 };
 
 // OSD inputs: the systematic generator as column words (bit r of gen_cols[c]
@@ -85,10 +89,12 @@ bool bp_shape_supported(const BpCodeDev& c);
 //   team            wavefronts cooperating on one codeword (1, 2, 4, 8, 16); 0 = default
 //   teams_per_block codewords in flight per workgroup for single-wave teams (1, 2, 4)
 //   max_waves       persistent-grid size in wavefronts (0 = 32 per CU)
+//   steal_chunk     0: static striding; > 0: work stealing in chunks of this many codewords
 struct BpTune {
     int team = 0;
     int teams_per_block = 4;
     int max_waves = 0;
+    int steal_chunk = 0;
 };
 bool bp_team_supported(const BpCodeDev& c, int team);
 int bp_default_team(const BpCodeDev& c);

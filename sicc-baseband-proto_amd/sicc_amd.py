@@ -237,22 +237,23 @@ class Decoder:
         check(self.lib.sicc_decode_device(self.handle, C.c_void_p(d_llr), cw, d_info, d_code, d_app, d_ok, d_used,
                                           d_iters, d_dist, stream))
 
-    def set_tune(self, team: int = 0, teams_per_block: int = 4, max_waves: int = 0):
-        self.lib.sicc_set_bp_tune.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
-        check(self.lib.sicc_set_bp_tune(self.handle, team, teams_per_block, max_waves))
+    def set_tune(self, team: int = 0, teams_per_block: int = 4, max_waves: int = 0, steal_chunk: int = 0):
+        self.lib.sicc_set_bp_tune.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.c_int]
+        check(self.lib.sicc_set_bp_tune(self.handle, team, teams_per_block, max_waves, steal_chunk))
 
     def get_tune(self):
-        t, b, w = C.c_int(), C.c_int(), C.c_int()
-        self.lib.sicc_get_bp_tune.argtypes = [C.c_ulonglong, i32p, i32p, i32p]
-        check(self.lib.sicc_get_bp_tune(self.handle, C.byref(t), C.byref(b), C.byref(w)))
-        return t.value, b.value, w.value
+        t, b, w, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.lib.sicc_get_bp_tune.argtypes = [C.c_ulonglong, i32p, i32p, i32p, i32p]
+        check(self.lib.sicc_get_bp_tune(self.handle, C.byref(t), C.byref(b), C.byref(w), C.byref(s)))
+        return {"team": t.value, "teams_per_block": b.value, "max_waves": w.value, "steal_chunk": s.value}
 
     def autotune(self, d_llr: int, cw: int):
-        t, b, w, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
-        self.lib.sicc_autotune.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, i32p, i32p, i32p, f64p]
+        t, b, w, s, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self.lib.sicc_autotune.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, i32p, i32p, i32p, i32p, f64p]
         check(self.lib.sicc_autotune(self.handle, C.c_void_p(d_llr), cw, C.byref(t), C.byref(b), C.byref(w),
-                                     C.byref(ms)))
-        return {"team": t.value, "teams_per_block": b.value, "max_waves": w.value, "bp_ms": ms.value}
+                                     C.byref(s), C.byref(ms)))
+        return {"team": t.value, "teams_per_block": b.value, "max_waves": w.value, "steal_chunk": s.value,
+                "bp_ms": ms.value}
 
     def set_timing(self, on: bool):
         check(self.lib.sicc_set_kernel_timing(self.handle, int(on)))
@@ -311,6 +312,42 @@ class Decoder:
         s = StreamStats()
         check(self.lib.cuda_osd_get_stats(self.handle, C.byref(s)))
         return s
+
+
+class Group(C.Structure):
+    _fields_ = [("member", C.c_int), ("count", C.c_int), ("d_llr", C.c_void_p), ("d_info", C.c_void_p),
+                ("d_bp_ok", C.c_void_p), ("d_iterations", C.c_void_p), ("d_used_osd", C.c_void_p)]
+
+
+class MultiDecoder:
+    """Mixed-code batch decoder over existing Decoder objects (sicc_multi_* ABI)."""
+
+    def __init__(self, decoders):
+        self.lib = load()
+        self.decoders = list(decoders)
+        hs = (C.c_ulonglong * len(self.decoders))(*[d.handle for d in self.decoders])
+        h = C.c_ulonglong(0)
+        self.lib.sicc_multi_create.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, C.POINTER(C.c_ulonglong)]
+        self.lib.sicc_multi_destroy.argtypes = [C.c_ulonglong]
+        self.lib.sicc_multi_decode_device.argtypes = [C.c_ulonglong, C.POINTER(Group), C.c_int, C.c_int,
+                                                      C.c_void_p]
+        check(self.lib.sicc_multi_create(hs, len(self.decoders), C.byref(h)))
+        self.handle = h.value
+
+    def decode(self, groups, use_graph=False, stream=None):
+        arr = (Group * len(groups))(*groups)
+        check(self.lib.sicc_multi_decode_device(self.handle, arr, len(groups), int(use_graph), stream))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.sicc_multi_destroy(self.handle))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class OfdmConfig(C.Structure):
