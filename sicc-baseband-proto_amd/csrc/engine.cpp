@@ -1,5 +1,5 @@
-// decoder_engine.cpp -- device tables and BP -> OSD orchestration for one code.
-#include "decoder_engine.hpp"
+// engine.cpp -- device tables and BP -> OSD orchestration for one code.
+#include "engine.hpp"
 
 #include <algorithm>
 #include <stdexcept>
@@ -70,7 +70,7 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
         osddev_.gen_cols = gen_cols_.get();
         osddev_.info_cols = info_cols_.get();
     }
-    osd_count_.alloc(2);
+    own_.ensure(1, n);
 
     int dev = 0, cus = 256;
     SICC_HIP(hipGetDevice(&dev));
@@ -78,19 +78,21 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
     max_waves_ = std::max(cus, 1) * 32;
 }
 
-void Engine::ensure_scratch(int num_cw)
+void DecodeScratch::ensure(int num_cw, int n)
 {
-    if (num_cw <= scratch_cw_) return;
+    if (counters.size() < 2) counters.alloc(2);
+    if (num_cw <= cap) return;
     // Grow geometrically so a sweep of batch sizes does not reallocate each call.
-    const int cap = std::max(num_cw, scratch_cw_ * 2);
-    osd_llr_.alloc(static_cast<size_t>(cap) * code_.n);
-    osd_index_.alloc(cap);
-    scratch_cw_ = cap;
+    const int c = std::max(num_cw, cap * 2);
+    osd_llr.alloc(static_cast<size_t>(c) * n);
+    osd_index.alloc(c);
+    cap = c;
 }
 
 void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
-                    const DecodeOut& out, hipStream_t s)
+                    const DecodeOut& out, hipStream_t s, DecodeScratch* scratch)
 {
+    DecodeScratch& sc = scratch ? *scratch : own_;
     if (num_cw <= 0) return;
     if (!d_llr) throw std::invalid_argument("null LLR pointer");
     const bool use_osd = osd_order >= 0;
@@ -108,15 +110,15 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
     // counters: [0] OSD hand-off slots, [1] work-stealing cursor; cleared by a
     // kernel (no memset node when this sequence is captured in a graph)
     const bool steal = tune_.steal_chunk > 0;
-    if (use_osd || steal) launch_zero_i32(osd_count_.get(), 2, s);
+    sc.ensure(use_osd ? num_cw : 0, code_.n);
+    if (use_osd || steal) launch_zero_i32(sc.counters.get(), 2, s);
     if (use_osd) {
-        ensure_scratch(num_cw);
-        io.osd_llr = osd_llr_.get();
-        io.osd_index = osd_index_.get();
-        io.osd_count = osd_count_.get();
+        io.osd_llr = sc.osd_llr.get();
+        io.osd_index = sc.osd_index.get();
+        io.osd_count = sc.counters.get();
     }
     if (steal) {
-        io.work_ctr = osd_count_.get() + 1;
+        io.work_ctr = sc.counters.get() + 1;
         io.chunk = tune_.steal_chunk;
     }
     timed_begin(0, s);
@@ -126,9 +128,9 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         OsdParams op = osd;
         op.order = osd_order;
         OsdIo oio;
-        oio.llr = osd_llr_.get();
-        oio.index = osd_index_.get();
-        oio.count = osd_count_.get();
+        oio.llr = sc.osd_llr.get();
+        oio.index = sc.osd_index.get();
+        oio.count = sc.counters.get();
         oio.code_bits = out.code;
         oio.info_bits = out.info;
         oio.distance = out.distance;
@@ -197,11 +199,11 @@ double Engine::autotune(const float* d_llr, int num_cw, const BpParams& bp, hipS
                         t.steal_chunk = sc;
                         BpIo tio = io;
                         if (sc > 0) {
-                            tio.work_ctr = osd_count_.get() + 1;
+                            tio.work_ctr = own_.counters.get() + 1;
                             tio.chunk = sc;
                         }
                         auto run = [&] {
-                            if (sc > 0) launch_zero_i32(osd_count_.get() + 1, 1, s);
+                            if (sc > 0) launch_zero_i32(own_.counters.get() + 1, 1, s);
                             launch_bp_flood(bpdev_, bp, tio, max_waves_, s, t);
                         };
                         run();  // warm-up

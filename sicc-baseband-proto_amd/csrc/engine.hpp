@@ -1,4 +1,4 @@
-// decoder_engine.hpp -- one LDPC code resident on the GPU: graph tables for BP,
+// engine.hpp -- one LDPC code resident on the GPU: graph tables for BP,
 // the systematic generator for OSD, and the device-side BP -> OSD hand-off.
 // Everything here is stream-ordered and asynchronous; nothing synchronizes
 // the device.
@@ -22,6 +22,17 @@ struct DecodeOut {
     float* distance = nullptr;    // [cw] OSD metric, 0 for BP-accepted frames
 };
 
+// Per-caller device scratch of a decode: the BP -> OSD hand-off buffers and
+// the two counters ([0] hand-off slots, [1] work-stealing cursor).  Decodes
+// that may run concurrently on different streams need separate scratch.
+struct DecodeScratch {
+    DevBuf<float> osd_llr;
+    DevBuf<int> osd_index;
+    DevBuf<int> counters;
+    int cap = 0;
+    void ensure(int num_cw, int n);
+};
+
 class Engine {
 public:
     explicit Engine(LdpcCode code);
@@ -34,14 +45,14 @@ public:
     // BP over num_cw device-resident codewords, then OSD (order >= 0) on the
     // frames BP did not accept.  osd_order < 0 disables OSD.
     void decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
-                const DecodeOut& out, hipStream_t s);
+                const DecodeOut& out, hipStream_t s, DecodeScratch* scratch = nullptr);
     // OSD on every codeword (no BP).
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 
     int max_waves() const { return max_waves_; }
     // Pre-size the OSD hand-off scratch for batches of up to num_cw codewords
     // (decode() grows it on demand, which a graph capture cannot do).
-    void reserve(int num_cw) { ensure_scratch(num_cw); }
+    void reserve(int num_cw) { own_.ensure(num_cw, code_.n); }
 
     // BP launch shape.  autotune() times every supported shape (wavefronts per
     // codeword, codewords per workgroup, persistent grid size) on a sample batch
@@ -74,7 +85,6 @@ private:
     int t_cnt_[2] = {0, 0};
 
 private:
-    void ensure_scratch(int num_cw);
 
     LdpcCode code_;
     bool osd_ok_ = false;
@@ -83,10 +93,7 @@ private:
     DevBuf<int> col_meta_;
     DevBuf<int> info_cols_;
     DevBuf<unsigned long long> gen_cols_;
-    DevBuf<float> osd_llr_;
-    DevBuf<int> osd_index_;
-    DevBuf<int> osd_count_;
-    int scratch_cw_ = 0;
+    DecodeScratch own_;
     BpCodeDev bpdev_;
     OsdCodeDev osddev_;
 };
