@@ -1,0 +1,77 @@
+/*
+ * sicc_serve_c_api.h -- continuous-batching decode server in libsicc_amd.so.
+ *
+ * No counterpart in the reference ABI; it generalizes the reference's
+ * cross-frame batching loop (staging of frames into one decoder submit and
+ * out-of-order completion by (frame id, block index),
+ * source/cpp_cuda/uhd_cpp/src/uhd_ldpc_ofdm_link.cpp:3309-3432 and 11482-11633)
+ * to jobs of any size and mixed codes.
+ *
+ * A server owns a device-memory pool of `budget_bytes`.  A submitted job
+ * (codewords of one member code) is admitted, in arrival order, as soon as its
+ * LLR, result and scratch space fits the pool; its LLRs then stay in HBM until
+ * the job completes.  A scheduler thread keeps up to `slots` launches in
+ * flight: whenever a slot is free it packs every admitted, not yet launched
+ * job (up to `max_launch_codewords`) into per-code groups and launches them, so
+ * jobs that arrive while others run overlap with them on the device.
+ * slots = 1 and max_jobs_per_launch = 1 give the plain one-job-per-launch server.
+ * Results are identical to decoding each job alone (per-codeword decoding is
+ * independent of batch composition).
+ * Status codes and cuda_osd_get_last_error() as in matlab_decoder_c_api.h.
+ */
+#ifndef SICC_SERVE_C_API_H
+#define SICC_SERVE_C_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Members are decoder handles (stream BP-OSD or BP CSR decoders); the server
+ * keeps them alive.  max_jobs_per_launch <= 0: unlimited. */
+int sicc_server_create(const unsigned long long* decoders, int count, unsigned long long budget_bytes, int slots,
+                       int max_jobs_per_launch, int max_launch_codewords, unsigned long long* out_handle);
+int sicc_server_destroy(unsigned long long handle);
+
+/* Queue a job.  llrs [count][n] of member `member`, host (device_ptr = 0, the
+ * data is copied before the call returns) or device memory (must stay valid
+ * until the job completes).  A job larger than the whole budget is rejected
+ * with CUDA_OSD_STATUS_INVALID_ARGUMENT. */
+int sicc_server_submit(unsigned long long handle, unsigned long long job_id, int member, const float* llrs,
+                       int count, int device_ptr);
+
+/* Ids of completed jobs (at most max_jobs), oldest completion first. */
+int sicc_server_poll(unsigned long long handle, int max_jobs, unsigned long long* job_ids, int* out_count);
+
+/* Results of a completed job (then forgotten): info [count][k], BP success and
+ * OSD flags [count] (any may be null), latency from submit to completion. */
+int sicc_server_fetch(unsigned long long handle, unsigned long long job_id, unsigned char* info_bits,
+                      unsigned char* bp_success, unsigned char* used_osd, double* latency_us);
+
+/* Block until every submitted job has completed. */
+int sicc_server_drain(unsigned long long handle);
+
+/* Native load driver: submits job j at arrival_us[j] (microseconds after the
+ * call starts, busy-waiting on a steady clock) from device LLR sources, waits
+ * for all, and returns each job's latency and the wall time.  Results are
+ * discarded.  For benchmarks. */
+int sicc_server_drive(unsigned long long handle, int n_jobs, const int* member, const int* count,
+                      const unsigned long long* d_llr_ptrs, const double* arrival_us, double* latency_us,
+                      double* wall_s);
+
+typedef struct sicc_server_stats_t {
+    unsigned long long jobs_submitted;
+    unsigned long long jobs_completed;
+    unsigned long long codewords_completed;
+    unsigned long long launches;
+    unsigned long long max_inflight_launches;
+    unsigned long long pool_bytes;
+    unsigned long long pool_high_water;
+    double mean_jobs_per_launch;
+} sicc_server_stats_t;
+int sicc_server_stats(unsigned long long handle, sicc_server_stats_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SICC_SERVE_C_API_H */

@@ -1,0 +1,576 @@
+// server.cpp -- continuous-batching decode server (include/sicc_serve_c_api.h).
+//
+// Admission: jobs wait in arrival order until their device footprint (LLRs,
+// results, per-job scratch share) fits the pool; admitted jobs' LLRs are copied
+// into HBM and stay there until the job completes.  Scheduling: a thread keeps
+// up to `slots` launches in flight; each launch takes every admitted job not
+// yet launched (bounded by jobs and codewords per launch), packs jobs of the
+// same code contiguously (device-to-device), decodes each code group with its
+// persistent BP(+OSD) kernels on the slot's stream, scatters results back to
+// the jobs' buffers and copies them to pinned host memory.  Completion is seen
+// by polling the slot's event, so jobs arriving during a launch are admitted
+// and launched on the next free slot while earlier launches still run.
+#include "../../include/sicc_serve_c_api.h"
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "decoder_handles.hpp"
+
+namespace sicc {
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over [0, cap) with coalescing.
+class Pool {
+public:
+    explicit Pool(size_t cap) : cap_(cap) { free_[0] = cap; }
+    size_t alloc(size_t bytes)
+    {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_ += bytes;
+                high_ = std::max(high_, used_);
+                return off;
+            }
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t capacity() const { return cap_; }
+    size_t high_water() const { return high_; }
+
+private:
+    size_t cap_;
+    size_t used_ = 0;
+    size_t high_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Job {
+    uint64_t id = 0;
+    int member = 0;
+    int count = 0;
+    const float* src = nullptr;
+    bool src_dev = false;
+    std::vector<float> host_copy;
+    uint64_t t_submit = 0, t_done = 0;
+    size_t off = SIZE_MAX, bytes = 0;
+    float* d_llr = nullptr;
+    uint8_t* d_info = nullptr;
+    uint8_t* d_ok = nullptr;
+    uint8_t* d_used = nullptr;
+    size_t res_off = 0;  // offset into the slot's pinned results
+    std::vector<uint8_t> info, ok, used;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<std::shared_ptr<Job>> jobs;
+    std::vector<DevBuf<float>> pack_llr;
+    std::vector<DevBuf<uint8_t>> pack_out;  // info | ok | used per member
+    std::vector<DecodeScratch> scratch;
+    uint8_t* h_res = nullptr;
+    size_t h_res_bytes = 0;
+};
+
+class Server {
+public:
+    Server(std::vector<std::shared_ptr<Decoder>> members, size_t budget, int slots, int max_jobs, int max_cw)
+        : members_(std::move(members)), pool_(budget), max_jobs_(max_jobs), max_cw_(max_cw)
+    {
+        SICC_HIP(hipGetDevice(&device_));
+        SICC_HIP(hipMalloc(reinterpret_cast<void**>(&base_), budget));
+        SICC_HIP(hipStreamCreateWithFlags(&admit_stream_, hipStreamNonBlocking));
+        SICC_HIP(hipEventCreateWithFlags(&admit_event_, hipEventDisableTiming));
+        size_t kmax = 0;
+        for (auto& m : members_) kmax = std::max<size_t>(kmax, m->eng->code().k);
+        slots_.resize(std::max(slots, 1));
+        for (auto& s : slots_) {
+            SICC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+            SICC_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+            s.pack_llr.resize(members_.size());
+            s.pack_out.resize(members_.size());
+            s.scratch.resize(members_.size());
+            s.h_res_bytes = static_cast<size_t>(max_cw_) * (kmax + 2);
+            SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_res), s.h_res_bytes, hipHostMallocDefault));
+        }
+        thread_ = std::thread([this] { loop(); });
+    }
+
+    ~Server()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (thread_.joinable()) thread_.join();
+        for (auto& s : slots_) {
+            if (s.stream) (void)hipStreamSynchronize(s.stream);
+            if (s.h_res) (void)hipHostFree(s.h_res);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+        }
+        if (admit_stream_) {
+            (void)hipStreamSynchronize(admit_stream_);
+            (void)hipStreamDestroy(admit_stream_);
+        }
+        if (admit_event_) (void)hipEventDestroy(admit_event_);
+        if (base_) (void)hipFree(base_);
+    }
+
+    size_t footprint(int member, int count) const
+    {
+        const LdpcCode& c = members_[member]->eng->code();
+        const size_t n = static_cast<size_t>(count);
+        // LLRs + info + flags, plus the OSD hand-off share of a packed launch
+        size_t b = align_up(n * c.n * sizeof(float)) + align_up(n * c.k) + 2 * align_up(n);
+        if (members_[member]->kind == Kind::Stream && members_[member]->osd_order >= 0) b += align_up(n * c.n * 4);
+        return b;
+    }
+
+    void submit(uint64_t id, int member, const float* llr, int count, bool dev)
+    {
+        if (member < 0 || member >= static_cast<int>(members_.size())) throw std::invalid_argument("submit: bad member");
+        if (!llr || count <= 0) throw std::invalid_argument("submit: empty job");
+        if (count > max_cw_) throw std::invalid_argument("submit: job larger than max_launch_codewords");
+        if (footprint(member, count) > pool_.capacity())
+            throw std::invalid_argument("submit: job larger than the device-memory budget");
+        auto j = std::make_shared<Job>();
+        j->id = id;
+        j->member = member;
+        j->count = count;
+        j->src_dev = dev;
+        if (dev) {
+            j->src = llr;
+        } else {
+            const size_t n = static_cast<size_t>(count) * members_[member]->eng->code().n;
+            j->host_copy.assign(llr, llr + n);
+            j->src = j->host_copy.data();
+        }
+        j->t_submit = now_us();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (known_.count(id)) throw std::invalid_argument("submit: job id already in use");
+            known_[id] = j;
+            arrivals_.push_back(j);
+            ++st_.jobs_submitted;
+        }
+        cv_.notify_all();
+    }
+
+    int poll(int max, unsigned long long* ids)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        int got = 0;
+        while (got < max && !completed_.empty()) {
+            ids[got++] = completed_.front();
+            completed_.pop_front();
+        }
+        return got;
+    }
+
+    void fetch(uint64_t id, uint8_t* info, uint8_t* ok, uint8_t* used, double* lat)
+    {
+        std::shared_ptr<Job> j;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = known_.find(id);
+            if (it == known_.end() || it->second->t_done == 0) throw std::invalid_argument("fetch: job not completed");
+            j = it->second;
+            known_.erase(it);
+        }
+        if (info) std::memcpy(info, j->info.data(), j->info.size());
+        if (ok) std::memcpy(ok, j->ok.data(), j->ok.size());
+        if (used) std::memcpy(used, j->used.data(), j->used.size());
+        if (lat) *lat = static_cast<double>(j->t_done - j->t_submit);
+    }
+
+    void drain()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return st_.jobs_completed == st_.jobs_submitted || failed_; });
+        if (failed_) throw std::runtime_error("server: " + fail_msg_);
+    }
+
+    double job_latency_and_forget(uint64_t id)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = known_.find(id);
+        if (it == known_.end()) return -1.0;
+        const double l = static_cast<double>(it->second->t_done - it->second->t_submit);
+        known_.erase(it);
+        return l;
+    }
+
+    void clear_completed()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        completed_.clear();
+    }
+
+    sicc_server_stats_t stats()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        sicc_server_stats_t s = st_;
+        s.pool_bytes = pool_.capacity();
+        s.pool_high_water = pool_.high_water();
+        s.mean_jobs_per_launch = st_.launches ? static_cast<double>(st_.jobs_completed) / st_.launches : 0.0;
+        return s;
+    }
+
+    int device() const { return device_; }
+
+private:
+    void loop()
+    {
+        try {
+            SICC_HIP(hipSetDevice(device_));
+            std::unique_lock<std::mutex> lk(mu_);
+            while (!stop_) {
+                bool progress = harvest();
+                progress |= admit();
+                progress |= launch();
+                const bool inflight = std::any_of(slots_.begin(), slots_.end(), [](const Slot& s) { return s.busy; });
+                if (!progress) {
+                    if (inflight || !arrivals_.empty() || !resident_.empty()) {
+                        lk.unlock();
+                        std::this_thread::yield();
+                        lk.lock();
+                    } else {
+                        cv_.wait(lk, [&] { return stop_ || !arrivals_.empty(); });
+                    }
+                }
+            }
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> g(mu_);
+            failed_ = true;
+            fail_msg_ = e.what();
+            cv_done_.notify_all();
+        }
+    }
+
+    bool harvest()
+    {
+        bool any = false;
+        for (auto& s : slots_) {
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) throw_hip(q, "server launch");
+            const uint64_t t = now_us();
+            for (auto& j : s.jobs) {
+                const int k = members_[j->member]->eng->code().k;
+                const uint8_t* r = s.h_res + j->res_off;
+                j->info.assign(r, r + static_cast<size_t>(j->count) * k);
+                r += static_cast<size_t>(j->count) * k;
+                j->ok.assign(r, r + j->count);
+                j->used.assign(r + j->count, r + 2 * j->count);
+                j->t_done = t;
+                pool_.release(j->off, j->bytes);
+                completed_.push_back(j->id);
+                ++st_.jobs_completed;
+                st_.codewords_completed += static_cast<unsigned long long>(j->count);
+            }
+            s.jobs.clear();
+            s.busy = false;
+            any = true;
+        }
+        if (any) cv_done_.notify_all();
+        return any;
+    }
+
+    bool admit()
+    {
+        bool any = false;
+        while (!arrivals_.empty()) {
+            auto j = arrivals_.front();
+            const size_t bytes = footprint(j->member, j->count);
+            const size_t off = pool_.alloc(bytes);
+            if (off == SIZE_MAX) break;  // in order: later jobs wait behind the head
+            arrivals_.pop_front();
+            const LdpcCode& c = members_[j->member]->eng->code();
+            const size_t n = static_cast<size_t>(j->count);
+            j->off = off;
+            j->bytes = bytes;
+            char* p = base_ + off;
+            j->d_llr = reinterpret_cast<float*>(p);
+            p += align_up(n * c.n * sizeof(float));
+            j->d_info = reinterpret_cast<uint8_t*>(p);
+            p += align_up(n * c.k);
+            j->d_ok = reinterpret_cast<uint8_t*>(p);
+            p += align_up(n);
+            j->d_used = reinterpret_cast<uint8_t*>(p);
+            SICC_HIP(hipMemcpyAsync(j->d_llr, j->src, n * c.n * sizeof(float),
+                                    j->src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, admit_stream_));
+            resident_.push_back(j);
+            any = true;
+        }
+        if (any) SICC_HIP(hipEventRecord(admit_event_, admit_stream_));
+        return any;
+    }
+
+    bool launch()
+    {
+        bool any = false;
+        for (auto& s : slots_) {
+            if (s.busy || resident_.empty()) continue;
+            // take jobs in admission order within the launch limits
+            int total = 0;
+            while (!resident_.empty()) {
+                auto& j = resident_.front();
+                if (max_jobs_ > 0 && static_cast<int>(s.jobs.size()) >= max_jobs_) break;
+                if (total + j->count > max_cw_) break;
+                total += j->count;
+                s.jobs.push_back(j);
+                resident_.pop_front();
+            }
+            if (s.jobs.empty()) continue;
+            SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
+            // group by member
+            std::vector<std::vector<std::shared_ptr<Job>>> by(members_.size());
+            for (auto& j : s.jobs) by[j->member].push_back(j);
+            size_t res = 0;
+            for (size_t m = 0; m < by.size(); ++m) {
+                if (by[m].empty()) continue;
+                Decoder& d = *members_[m];
+                const LdpcCode& c = d.eng->code();
+                const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
+                int cw = 0;
+                for (auto& j : by[m]) cw += j->count;
+                const float* llr = nullptr;
+                uint8_t *info = nullptr, *ok = nullptr, *used = nullptr;
+                if (by[m].size() == 1) {
+                    auto& j = by[m][0];
+                    llr = j->d_llr;
+                    info = j->d_info;
+                    ok = j->d_ok;
+                    used = j->d_used;
+                } else {
+                    if (s.pack_llr[m].size() < static_cast<size_t>(cw) * c.n) {
+                        s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
+                        s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));
+                    }
+                    size_t o = 0;
+                    for (auto& j : by[m]) {
+                        SICC_HIP(hipMemcpyAsync(s.pack_llr[m].get() + o * c.n, j->d_llr,
+                                                static_cast<size_t>(j->count) * c.n * sizeof(float),
+                                                hipMemcpyDeviceToDevice, s.stream));
+                        o += j->count;
+                    }
+                    llr = s.pack_llr[m].get();
+                    info = s.pack_out[m].get();
+                    ok = info + static_cast<size_t>(max_cw_) * c.k;
+                    used = ok + max_cw_;
+                }
+                DecodeOut out;
+                out.info = info;
+                out.bp_ok = ok;
+                out.used_osd = used;
+                d.eng->decode(llr, cw, d.bp, osd, d.osd, out, s.stream, &s.scratch[m]);
+                if (osd < 0) launch_zero_u8(used, cw, s.stream);
+                // results -> pinned host staging, per job [info | ok | used]
+                size_t o = 0;
+                for (auto& j : by[m]) {
+                    const size_t n = static_cast<size_t>(j->count);
+                    j->res_off = res;
+                    SICC_HIP(hipMemcpyAsync(s.h_res + res, info + o * c.k, n * c.k, hipMemcpyDeviceToHost, s.stream));
+                    SICC_HIP(hipMemcpyAsync(s.h_res + res + n * c.k, ok + o, n, hipMemcpyDeviceToHost, s.stream));
+                    SICC_HIP(hipMemcpyAsync(s.h_res + res + n * c.k + n, used + o, n, hipMemcpyDeviceToHost, s.stream));
+                    res += n * (c.k + 2);
+                    o += n;
+                }
+            }
+            SICC_HIP(hipEventRecord(s.done, s.stream));
+            s.busy = true;
+            ++st_.launches;
+            const auto inflight = static_cast<unsigned long long>(
+                std::count_if(slots_.begin(), slots_.end(), [](const Slot& x) { return x.busy; }));
+            st_.max_inflight_launches = std::max(st_.max_inflight_launches, inflight);
+            any = true;
+        }
+        return any;
+    }
+
+    static void launch_zero_u8(uint8_t* p, int n, hipStream_t s)
+    {
+        // BP-only members report used_osd = 0 (this sequence is never graph-captured)
+        SICC_HIP(hipMemsetAsync(p, 0, static_cast<size_t>(n), s));
+    }
+
+    std::vector<std::shared_ptr<Decoder>> members_;
+    Pool pool_;
+    int max_jobs_;
+    int max_cw_;
+    int device_ = 0;
+    char* base_ = nullptr;
+    hipStream_t admit_stream_ = nullptr;
+    hipEvent_t admit_event_ = nullptr;
+    std::vector<Slot> slots_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    std::deque<std::shared_ptr<Job>> arrivals_, resident_;
+    std::deque<unsigned long long> completed_;
+    std::unordered_map<uint64_t, std::shared_ptr<Job>> known_;
+    sicc_server_stats_t st_{};
+    bool stop_ = false;
+    bool failed_ = false;
+    std::string fail_msg_;
+    std::thread thread_;
+};
+
+HandleTable<Server>& stable()
+{
+    static HandleTable<Server> t;
+    return t;
+}
+
+std::shared_ptr<Server> slookup(unsigned long long h)
+{
+    auto s = stable().get(h);
+    if (!s) throw std::invalid_argument("server: unknown handle");
+    SICC_HIP(hipSetDevice(s->device()));
+    return s;
+}
+
+}  // namespace
+}  // namespace sicc
+
+using namespace sicc;
+
+extern "C" {
+
+int sicc_server_create(const unsigned long long* decoders, int count, unsigned long long budget_bytes, int slots,
+                       int max_jobs_per_launch, int max_launch_codewords, unsigned long long* out_handle)
+{
+    return guarded([&] {
+        if (!out_handle || !decoders || count <= 0) throw std::invalid_argument("server_create: no decoders");
+        *out_handle = 0;
+        if (budget_bytes < (1u << 20)) throw std::invalid_argument("server_create: budget below 1 MiB");
+        if (slots <= 0 || max_launch_codewords <= 0) throw std::invalid_argument("server_create: bad limits");
+        std::vector<std::shared_ptr<Decoder>> members;
+        for (int i = 0; i < count; ++i) {
+            auto d = table().get(decoders[i]);
+            if (!d || d->kind == Kind::OsdOnly) throw std::invalid_argument("server_create: members must be BP decoders");
+            members.push_back(d);
+        }
+        *out_handle = stable().add(std::make_unique<Server>(std::move(members), static_cast<size_t>(budget_bytes), slots,
+                                                            max_jobs_per_launch, max_launch_codewords));
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_destroy(unsigned long long handle)
+{
+    return guarded([&] {
+        if (!stable().remove(handle)) throw std::invalid_argument("server: unknown handle");
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_submit(unsigned long long handle, unsigned long long job_id, int member, const float* llrs, int count,
+                       int device_ptr)
+{
+    return guarded([&] {
+        slookup(handle)->submit(job_id, member, llrs, count, device_ptr != 0);
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_poll(unsigned long long handle, int max_jobs, unsigned long long* job_ids, int* out_count)
+{
+    return guarded([&] {
+        if (!out_count || (max_jobs > 0 && !job_ids)) throw std::invalid_argument("server_poll: null output");
+        *out_count = max_jobs > 0 ? slookup(handle)->poll(max_jobs, job_ids) : 0;
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_fetch(unsigned long long handle, unsigned long long job_id, unsigned char* info_bits,
+                      unsigned char* bp_success, unsigned char* used_osd, double* latency_us)
+{
+    return guarded([&] {
+        slookup(handle)->fetch(job_id, info_bits, bp_success, used_osd, latency_us);
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_drain(unsigned long long handle)
+{
+    return guarded([&] {
+        slookup(handle)->drain();
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_drive(unsigned long long handle, int n_jobs, const int* member, const int* count,
+                      const unsigned long long* d_llr_ptrs, const double* arrival_us, double* latency_us,
+                      double* wall_s)
+{
+    return guarded([&] {
+        auto s = slookup(handle);
+        if (n_jobs <= 0 || !member || !count || !d_llr_ptrs || !arrival_us)
+            throw std::invalid_argument("server_drive: empty schedule");
+        const uint64_t base_id = 0x8000000000000000ull + (now_us() << 20);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int j = 0; j < n_jobs; ++j) {
+            const auto due = t0 + std::chrono::nanoseconds(static_cast<long long>(arrival_us[j] * 1000.0));
+            while (std::chrono::steady_clock::now() < due) {
+            }
+            s->submit(base_id + j, member[j], reinterpret_cast<const float*>(d_llr_ptrs[j]), count[j], true);
+        }
+        s->drain();
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int j = 0; j < n_jobs; ++j) {
+            const double l = s->job_latency_and_forget(base_id + j);
+            if (latency_us) latency_us[j] = l;
+        }
+        s->clear_completed();
+        if (wall_s) *wall_s = wall;
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_stats(unsigned long long handle, sicc_server_stats_t* out)
+{
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("server_stats: null output");
+        *out = slookup(handle)->stats();
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+}  // extern "C"

@@ -350,6 +350,91 @@ class MultiDecoder:
         self.close()
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_ulonglong), ("jobs_completed", C.c_ulonglong),
+                ("codewords_completed", C.c_ulonglong), ("launches", C.c_ulonglong),
+                ("max_inflight_launches", C.c_ulonglong), ("pool_bytes", C.c_ulonglong),
+                ("pool_high_water", C.c_ulonglong), ("mean_jobs_per_launch", C.c_double)]
+
+
+class Server:
+    """Continuous-batching decode server (sicc_server_* ABI)."""
+
+    def __init__(self, decoders, budget_bytes=16 << 30, slots=4, max_jobs_per_launch=0, max_launch_codewords=65536):
+        self.lib = load()
+        lib = self.lib
+        lib.sicc_server_create.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, C.c_ulonglong, C.c_int, C.c_int,
+                                           C.c_int, C.POINTER(C.c_ulonglong)]
+        lib.sicc_server_destroy.argtypes = [C.c_ulonglong]
+        lib.sicc_server_submit.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_int, C.c_void_p, C.c_int, C.c_int]
+        lib.sicc_server_poll.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(C.c_ulonglong), i32p]
+        lib.sicc_server_fetch.argtypes = [C.c_ulonglong, C.c_ulonglong, u8p, u8p, u8p, f64p]
+        lib.sicc_server_drain.argtypes = [C.c_ulonglong]
+        lib.sicc_server_drive.argtypes = [C.c_ulonglong, C.c_int, i32p, i32p, C.POINTER(C.c_ulonglong), f64p, f64p,
+                                          f64p]
+        lib.sicc_server_stats.argtypes = [C.c_ulonglong, C.POINTER(ServerStats)]
+        self.decoders = list(decoders)
+        hs = (C.c_ulonglong * len(self.decoders))(*[d.handle for d in self.decoders])
+        h = C.c_ulonglong(0)
+        check(lib.sicc_server_create(hs, len(self.decoders), budget_bytes, slots, max_jobs_per_launch,
+                                     max_launch_codewords, C.byref(h)))
+        self.handle = h.value
+
+    def submit(self, job_id, member, llr, device_ptr=None, count=None):
+        if device_ptr is not None:
+            check(self.lib.sicc_server_submit(self.handle, job_id, member, C.c_void_p(device_ptr), count, 1))
+        else:
+            llr = np.ascontiguousarray(llr, np.float32)
+            check(self.lib.sicc_server_submit(self.handle, job_id, member, llr.ctypes.data_as(C.c_void_p),
+                                              llr.shape[0], 0))
+
+    def poll(self, max_jobs=1024):
+        ids = (C.c_ulonglong * max_jobs)()
+        n = C.c_int(0)
+        check(self.lib.sicc_server_poll(self.handle, max_jobs, ids, C.byref(n)))
+        return [ids[i] for i in range(n.value)]
+
+    def fetch(self, job_id, count, k):
+        info = np.zeros((count, k), np.uint8)
+        ok = np.zeros(count, np.uint8)
+        used = np.zeros(count, np.uint8)
+        lat = C.c_double(0)
+        check(self.lib.sicc_server_fetch(self.handle, job_id, ptr(info, u8p), ptr(ok, u8p), ptr(used, u8p),
+                                         C.byref(lat)))
+        return info, ok, used, lat.value
+
+    def drain(self):
+        check(self.lib.sicc_server_drain(self.handle))
+
+    def drive(self, members, counts, d_ptrs, arrivals_us):
+        n = len(members)
+        mem = np.ascontiguousarray(members, np.int32)
+        cnt = np.ascontiguousarray(counts, np.int32)
+        ptrs = (C.c_ulonglong * n)(*[int(p) for p in d_ptrs])
+        arr = np.ascontiguousarray(arrivals_us, np.float64)
+        lat = np.zeros(n, np.float64)
+        wall = C.c_double(0)
+        check(self.lib.sicc_server_drive(self.handle, n, ptr(mem, i32p), ptr(cnt, i32p), ptrs, ptr(arr, f64p),
+                                         ptr(lat, f64p), C.byref(wall)))
+        return lat, wall.value
+
+    def stats(self):
+        s = ServerStats()
+        check(self.lib.sicc_server_stats(self.handle, C.byref(s)))
+        return s
+
+    def close(self):
+        if self.handle:
+            check(self.lib.sicc_server_destroy(self.handle))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class OfdmConfig(C.Structure):
     _fields_ = [
         ("nfft", C.c_int), ("cp", C.c_int), ("preamble_len", C.c_int), ("pre_half_len", C.c_int),

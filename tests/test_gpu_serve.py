@@ -1,0 +1,114 @@
+"""Continuous-batching server: results alone == results in the mix (bit-exact),
+admission bounded by the memory budget, plain and batched modes, native driver."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import sicc_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    decs, codes = [], []
+    st = sa.Decoder.stream(sa.stream_config(max_iter=30, alpha=0.75))
+    st.set_osd(1, 64, 4096)
+    n_, m_, rp, ci = oracle.alist_parse(sa.CCSDS_128_64)
+    decs.append(st)
+    codes.append(oracle.Code(n_, m_, rp, ci, True))
+    for n, mb in [(256, 8), (1024, 4), (2048, 8)]:
+        m, rp, ci = sa.qc_code(n, mb, 3, seed=n + mb)
+        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75))
+        codes.append(oracle.Code(n, m, rp, ci, True))
+    return decs, codes
+
+
+def _jobs(codes, n_jobs, seed):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n_jobs):
+        mem = int(rng.integers(0, len(codes)))
+        cnt = int(rng.integers(1, 40))
+        c = codes[mem]
+        info = rng.integers(0, 2, (cnt, c.k), dtype=np.uint8)
+        jobs.append((mem, oracle.awgn_llr(c.encode(info), 2.0, c.k / c.n, rng)))
+    return jobs
+
+
+def _collect(srv, jobs, ids, codes):
+    out = {}
+    for i, (mem, llr) in zip(ids, jobs):
+        out[i] = srv.fetch(i, llr.shape[0], codes[mem].k)
+    return out
+
+
+def test_alone_equals_in_mix():
+    print("[case] server: alone vs mixed, 60 jobs over 4 codes", flush=True)
+    decs, codes = _members()
+    jobs = _jobs(codes, 60, 3)
+    with sa.Server(decs, budget_bytes=256 << 20, slots=4) as srv:
+        alone = {}
+        for j, (mem, llr) in enumerate(jobs):
+            srv.submit(1000 + j, mem, llr)
+            srv.drain()
+            alone[j] = srv.fetch(1000 + j, llr.shape[0], codes[mem].k)
+        for j, (mem, llr) in enumerate(jobs):  # all at once: packed, overlapped launches
+            srv.submit(5000 + j, mem, llr)
+        srv.drain()
+        done = srv.poll(4096)
+        assert set(range(5000, 5060)) <= set(done)
+        for j, (mem, llr) in enumerate(jobs):
+            info, ok, used, lat = srv.fetch(5000 + j, llr.shape[0], codes[mem].k)
+            a = alone[j]
+            assert np.array_equal(info, a[0]) and np.array_equal(ok, a[1]) and np.array_equal(used, a[2]), j
+            assert lat > 0
+        st = srv.stats()
+        assert st.jobs_completed == 120 and st.mean_jobs_per_launch > 1.0
+    # and the oracle agrees on BP-only members (the stream member is covered by the BP-OSD tests)
+    for j, (mem, llr) in enumerate(jobs[:20]):
+        if mem == 0:
+            continue
+        for i in range(llr.shape[0]):
+            o = codes[mem].nms(llr[i], max_iter=30, alpha=0.75)
+            assert np.array_equal(alone[j][0][i], o["hard"][codes[mem].info_cols])
+    for d in decs:
+        d.close()
+
+
+def test_budget_admission_and_plain_mode():
+    decs, codes = _members()
+    jobs = _jobs(codes, 30, 4)
+    # a tight budget forces in-order admission behind earlier jobs; plain mode = 1 job per launch
+    with sa.Server(decs, budget_bytes=2 << 20, slots=1, max_jobs_per_launch=1) as srv:
+        for j, (mem, llr) in enumerate(jobs):
+            srv.submit(j, mem, llr)
+        srv.drain()
+        st = srv.stats()
+        assert st.launches == 30 and st.max_inflight_launches == 1
+        assert st.pool_high_water <= 2 << 20
+        for j, (mem, llr) in enumerate(jobs):
+            srv.fetch(j, llr.shape[0], codes[mem].k)
+        big = np.zeros((512, 2048), np.float32)
+        with pytest.raises(sa.SiccError) as e:
+            srv.submit(999, 3, big)
+        assert e.value.status == sa.STATUS_INVALID_ARGUMENT
+    for d in decs:
+        d.close()
+
+
+def test_native_driver_latencies():
+    import torch
+
+    decs, codes = _members()
+    jobs = _jobs(codes, 80, 5)
+    dev = [torch.from_numpy(llr).cuda() for _, llr in jobs]
+    arrivals = np.cumsum(np.random.default_rng(6).exponential(50.0, len(jobs)))
+    with sa.Server(decs, budget_bytes=256 << 20, slots=4) as srv:
+        lat, wall = srv.drive([m for m, _ in jobs], [l.shape[0] for _, l in jobs], [t.data_ptr() for t in dev],
+                              arrivals)
+        assert (lat > 0).all() and wall > 0
+        assert srv.stats().jobs_completed == 80
+    for d in decs:
+        d.close()
