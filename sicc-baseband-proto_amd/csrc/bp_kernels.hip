@@ -54,6 +54,22 @@ __global__ void zero_i32_kernel(int* p, int count)
     if (i < count) p[i] = 0;
 }
 
+// One workgroup per table entry; float4 moves when both ends are 16-byte aligned.
+__global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* table, float* dst, int row_len)
+{
+    const GatherEntry e = table[blockIdx.x];
+    const size_t len = static_cast<size_t>(e.count) * row_len;
+    float* d = dst + static_cast<size_t>(e.dst_row) * row_len;
+    const bool vec = ((reinterpret_cast<uintptr_t>(e.src) | reinterpret_cast<uintptr_t>(d)) & 15u) == 0 && (len & 3u) == 0;
+    if (vec) {
+        const float4* s4 = reinterpret_cast<const float4*>(e.src);
+        float4* d4 = reinterpret_cast<float4*>(d);
+        for (size_t i = threadIdx.x; i < len / 4; i += blockDim.x) d4[i] = s4[i];
+    } else {
+        for (size_t i = threadIdx.x; i < len; i += blockDim.x) d[i] = e.src[i];
+    }
+}
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
 __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
                                                                               int e_pad)
@@ -365,6 +381,14 @@ void launch_zero_i32(int* p, int count, hipStream_t s)
 {
     if (count <= 0) return;
     hipLaunchKernelGGL(zero_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, s, p, count);
+}
+
+void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int row_len, hipStream_t s)
+{
+    if (entries <= 0) return;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(entries), dim3(256), 0, s, table, dst, row_len);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("gather_rows_kernel launch: ") + hipGetErrorString(e));
 }
 
 bool bp_team_supported(const BpCodeDev& c, int team)

@@ -81,6 +81,16 @@ struct OsdIo {
     int* status = nullptr;          // [frames] 0 ok, 1 rank-deficient generator
 };
 
+// Row gather for batch packing: entry i copies `count` rows of `row_len` floats
+// from `src` to rows [dst_row, dst_row + count) of the destination.  The table
+// may live in pinned host memory (read by the kernel over the bus).
+struct GatherEntry {
+    const float* src;
+    int count;
+    int dst_row;
+};
+void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int row_len, hipStream_t s);
+
 // Kernel launchers.  They validate shapes on the host and throw
 // std::invalid_argument when a code does not fit an instantiated variant.
 void launch_zero_i32(int* p, int count, hipStream_t s);

@@ -86,11 +86,10 @@ struct Job {
     std::vector<float> host_copy;
     uint64_t t_submit = 0, t_done = 0;
     size_t off = SIZE_MAX, bytes = 0;
-    float* d_llr = nullptr;
-    uint8_t* d_info = nullptr;
-    uint8_t* d_ok = nullptr;
-    uint8_t* d_used = nullptr;
-    size_t res_off = 0;  // offset into the slot's pinned results
+    const float* d_llr = nullptr;  // HBM-resident LLRs: the caller's device rows or a pool copy
+    size_t grp_base = 0;           // this job's group block in the slot's pinned results
+    int row = 0;                   // first row of the job inside its group
+    int grp_cw = 0;                // rows in the group
     std::vector<uint8_t> info, ok, used;
 };
 
@@ -99,11 +98,13 @@ struct Slot {
     hipEvent_t done = nullptr;
     bool busy = false;
     std::vector<std::shared_ptr<Job>> jobs;
-    std::vector<DevBuf<float>> pack_llr;
-    std::vector<DevBuf<uint8_t>> pack_out;  // info | ok | used per member
+    std::vector<DevBuf<float>> pack_llr;    // per member, max_cw rows
+    std::vector<DevBuf<uint8_t>> pack_out;  // per member, [info | ok | used] of one launch
     std::vector<DecodeScratch> scratch;
-    uint8_t* h_res = nullptr;
+    uint8_t* h_res = nullptr;  // pinned results of one launch
     size_t h_res_bytes = 0;
+    GatherEntry* table = nullptr;  // pinned gather table of one launch
+    int table_cap = 0;
 };
 
 class Server {
@@ -126,6 +127,9 @@ public:
             s.scratch.resize(members_.size());
             s.h_res_bytes = static_cast<size_t>(max_cw_) * (kmax + 2);
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_res), s.h_res_bytes, hipHostMallocDefault));
+            s.table_cap = max_cw_;  // at most one entry per codeword
+            SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.table), sizeof(GatherEntry) * s.table_cap,
+                                   hipHostMallocDefault));
         }
         thread_ = std::thread([this] { loop(); });
     }
@@ -141,6 +145,7 @@ public:
         for (auto& s : slots_) {
             if (s.stream) (void)hipStreamSynchronize(s.stream);
             if (s.h_res) (void)hipHostFree(s.h_res);
+            if (s.table) (void)hipHostFree(s.table);
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.stream) (void)hipStreamDestroy(s.stream);
         }
@@ -152,12 +157,13 @@ public:
         if (base_) (void)hipFree(base_);
     }
 
+    // Device memory a job holds from admission to completion: its LLR rows and
+    // its share of the packed launch (LLR rows, results, OSD hand-off rows).
     size_t footprint(int member, int count) const
     {
         const LdpcCode& c = members_[member]->eng->code();
         const size_t n = static_cast<size_t>(count);
-        // LLRs + info + flags, plus the OSD hand-off share of a packed launch
-        size_t b = align_up(n * c.n * sizeof(float)) + align_up(n * c.k) + 2 * align_up(n);
+        size_t b = 2 * align_up(n * c.n * sizeof(float)) + align_up(n * (c.k + 2));
         if (members_[member]->kind == Kind::Stream && members_[member]->osd_order >= 0) b += align_up(n * c.n * 4);
         return b;
     }
@@ -294,13 +300,15 @@ private:
             const uint64_t t = now_us();
             for (auto& j : s.jobs) {
                 const int k = members_[j->member]->eng->code().k;
-                const uint8_t* r = s.h_res + j->res_off;
-                j->info.assign(r, r + static_cast<size_t>(j->count) * k);
-                r += static_cast<size_t>(j->count) * k;
-                j->ok.assign(r, r + j->count);
-                j->used.assign(r + j->count, r + 2 * j->count);
+                const size_t n = static_cast<size_t>(j->count);
+                const uint8_t* g = s.h_res + j->grp_base;
+                const size_t gcw = static_cast<size_t>(j->grp_cw);
+                j->info.assign(g + static_cast<size_t>(j->row) * k, g + (static_cast<size_t>(j->row) + n) * k);
+                j->ok.assign(g + gcw * k + j->row, g + gcw * k + j->row + n);
+                j->used.assign(g + gcw * (k + 1) + j->row, g + gcw * (k + 1) + j->row + n);
                 j->t_done = t;
                 pool_.release(j->off, j->bytes);
+                j->host_copy = std::vector<float>();
                 completed_.push_back(j->id);
                 ++st_.jobs_completed;
                 st_.codewords_completed += static_cast<unsigned long long>(j->count);
@@ -315,31 +323,32 @@ private:
 
     bool admit()
     {
-        bool any = false;
+        bool any = false, copied = false;
         while (!arrivals_.empty()) {
             auto j = arrivals_.front();
             const size_t bytes = footprint(j->member, j->count);
             const size_t off = pool_.alloc(bytes);
             if (off == SIZE_MAX) break;  // in order: later jobs wait behind the head
             arrivals_.pop_front();
-            const LdpcCode& c = members_[j->member]->eng->code();
-            const size_t n = static_cast<size_t>(j->count);
             j->off = off;
             j->bytes = bytes;
-            char* p = base_ + off;
-            j->d_llr = reinterpret_cast<float*>(p);
-            p += align_up(n * c.n * sizeof(float));
-            j->d_info = reinterpret_cast<uint8_t*>(p);
-            p += align_up(n * c.k);
-            j->d_ok = reinterpret_cast<uint8_t*>(p);
-            p += align_up(n);
-            j->d_used = reinterpret_cast<uint8_t*>(p);
-            SICC_HIP(hipMemcpyAsync(j->d_llr, j->src, n * c.n * sizeof(float),
-                                    j->src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, admit_stream_));
+            if (j->src_dev) {
+                j->d_llr = j->src;  // already resident in HBM
+            } else {
+                const LdpcCode& c = members_[j->member]->eng->code();
+                float* dst = reinterpret_cast<float*>(base_ + off);
+                SICC_HIP(hipMemcpyAsync(dst, j->src, static_cast<size_t>(j->count) * c.n * sizeof(float),
+                                        hipMemcpyHostToDevice, admit_stream_));
+                j->d_llr = dst;
+                copied = true;
+            }
             resident_.push_back(j);
             any = true;
         }
-        if (any) SICC_HIP(hipEventRecord(admit_event_, admit_stream_));
+        if (copied) {
+            SICC_HIP(hipEventRecord(admit_event_, admit_stream_));
+            admit_pending_ = true;
+        }
         return any;
     }
 
@@ -359,60 +368,46 @@ private:
                 resident_.pop_front();
             }
             if (s.jobs.empty()) continue;
-            SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
-            // group by member
-            std::vector<std::vector<std::shared_ptr<Job>>> by(members_.size());
-            for (auto& j : s.jobs) by[j->member].push_back(j);
+            if (admit_pending_) SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
+            std::vector<std::vector<Job*>> by(members_.size());
+            for (auto& j : s.jobs) by[j->member].push_back(j.get());
             size_t res = 0;
+            int tbl = 0;
             for (size_t m = 0; m < by.size(); ++m) {
                 if (by[m].empty()) continue;
                 Decoder& d = *members_[m];
                 const LdpcCode& c = d.eng->code();
                 const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
                 int cw = 0;
-                for (auto& j : by[m]) cw += j->count;
+                for (Job* j : by[m]) {
+                    j->row = cw;
+                    cw += j->count;
+                }
                 const float* llr = nullptr;
-                uint8_t *info = nullptr, *ok = nullptr, *used = nullptr;
                 if (by[m].size() == 1) {
-                    auto& j = by[m][0];
-                    llr = j->d_llr;
-                    info = j->d_info;
-                    ok = j->d_ok;
-                    used = j->d_used;
+                    llr = by[m][0]->d_llr;
                 } else {
-                    if (s.pack_llr[m].size() < static_cast<size_t>(cw) * c.n) {
-                        s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
-                        s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));
-                    }
-                    size_t o = 0;
-                    for (auto& j : by[m]) {
-                        SICC_HIP(hipMemcpyAsync(s.pack_llr[m].get() + o * c.n, j->d_llr,
-                                                static_cast<size_t>(j->count) * c.n * sizeof(float),
-                                                hipMemcpyDeviceToDevice, s.stream));
-                        o += j->count;
-                    }
+                    if (s.pack_llr[m].size() == 0) s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
+                    GatherEntry* t = s.table + tbl;
+                    for (size_t i = 0; i < by[m].size(); ++i) t[i] = {by[m][i]->d_llr, by[m][i]->count, by[m][i]->row};
+                    launch_gather_rows(t, static_cast<int>(by[m].size()), s.pack_llr[m].get(), c.n, s.stream);
+                    tbl += static_cast<int>(by[m].size());
                     llr = s.pack_llr[m].get();
-                    info = s.pack_out[m].get();
-                    ok = info + static_cast<size_t>(max_cw_) * c.k;
-                    used = ok + max_cw_;
                 }
+                if (s.pack_out[m].size() == 0) s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));
+                uint8_t* o = s.pack_out[m].get();
                 DecodeOut out;
-                out.info = info;
-                out.bp_ok = ok;
-                out.used_osd = used;
+                out.info = o;
+                out.bp_ok = o + static_cast<size_t>(cw) * c.k;
+                out.used_osd = o + static_cast<size_t>(cw) * (c.k + 1);
                 d.eng->decode(llr, cw, d.bp, osd, d.osd, out, s.stream, &s.scratch[m]);
-                if (osd < 0) launch_zero_u8(used, cw, s.stream);
-                // results -> pinned host staging, per job [info | ok | used]
-                size_t o = 0;
-                for (auto& j : by[m]) {
-                    const size_t n = static_cast<size_t>(j->count);
-                    j->res_off = res;
-                    SICC_HIP(hipMemcpyAsync(s.h_res + res, info + o * c.k, n * c.k, hipMemcpyDeviceToHost, s.stream));
-                    SICC_HIP(hipMemcpyAsync(s.h_res + res + n * c.k, ok + o, n, hipMemcpyDeviceToHost, s.stream));
-                    SICC_HIP(hipMemcpyAsync(s.h_res + res + n * c.k + n, used + o, n, hipMemcpyDeviceToHost, s.stream));
-                    res += n * (c.k + 2);
-                    o += n;
+                const size_t bytes = static_cast<size_t>(cw) * (c.k + 2);
+                SICC_HIP(hipMemcpyAsync(s.h_res + res, o, bytes, hipMemcpyDeviceToHost, s.stream));
+                for (Job* j : by[m]) {
+                    j->grp_base = res;
+                    j->grp_cw = cw;
                 }
+                res += bytes;
             }
             SICC_HIP(hipEventRecord(s.done, s.stream));
             s.busy = true;
@@ -425,12 +420,6 @@ private:
         return any;
     }
 
-    static void launch_zero_u8(uint8_t* p, int n, hipStream_t s)
-    {
-        // BP-only members report used_osd = 0 (this sequence is never graph-captured)
-        SICC_HIP(hipMemsetAsync(p, 0, static_cast<size_t>(n), s));
-    }
-
     std::vector<std::shared_ptr<Decoder>> members_;
     Pool pool_;
     int max_jobs_;
@@ -439,6 +428,7 @@ private:
     char* base_ = nullptr;
     hipStream_t admit_stream_ = nullptr;
     hipEvent_t admit_event_ = nullptr;
+    bool admit_pending_ = false;
     std::vector<Slot> slots_;
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
