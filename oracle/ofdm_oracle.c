@@ -261,3 +261,141 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
     free(rot);
     free(hp);
 }
+
+/* ------------------------------------------------------------------ TX ----
+ * Restatement of the reference GPU transmitter,
+ * PIPE = source/cpp_cuda/uhd_cpp/src/gpu_ofdm_pipeline.cu:
+ *   ldpc_encode_generator_kernel            PIPE:129-150
+ *   DVB-S2 systematic copy / row sums / acc  PIPE:152-214
+ *   fill_resource_grid_kernel (+ precomp)    PIPE:216-257, qam_from_bits_dev
+ *   hash_u32 / uniform_01 / normal01         PIPE:259-280
+ *   apply_actual_channel_and_frequency_awgn  PIPE:282-314
+ *   cuFFT inverse (unnormalized) + compose   PIPE:316-342, PIPE:1032-1053
+ * Float operations in the reference's order; the inverse FFT in double. */
+
+static uint32_t tx_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+static float tx_uniform(uint32_t seed, uint32_t idx)
+{
+    const uint32_t h = tx_hash(seed ^ (idx * 0x9e3779b9u));
+    return ((float)h + 1.0f) * 2.3283064365386963e-10f;
+}
+
+static float tx_normal(uint32_t seed, uint32_t idx)
+{
+    float u1 = tx_uniform(seed, idx * 2u);
+    if (u1 < 1.0e-7f) u1 = 1.0e-7f;
+    const float u2 = tx_uniform(seed, idx * 2u + 1u);
+    return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * 3.14159265358979323846f * u2);
+}
+
+int oracle_tx_frame(const uint8_t* info, int n, int k, int blocks, const uint8_t* gen_kn, int dvb_s2,
+                    const int* row_info_offsets, const int* row_info_cols, int coded_bits, int nfft, int cp,
+                    int pre_len, int num_symbols, int active_sc, int bits_per_symbol, const int* pilot_index,
+                    const int* data_index, const int* used_shifted, const float* pilot_grid,
+                    const float* preamble, const float* constellation, const float* precomp_h,
+                    const float* actual_h, float amplitude, float noise_var, uint64_t seed,
+                    uint8_t* coded_out, float* frame_out)
+{
+    if (k <= 0 || n <= k || blocks <= 0 || blocks * n > coded_bits || nfft <= 0 || (nfft & (nfft - 1)) != 0)
+        return -1;
+    if (!dvb_s2 && !gen_kn) return -1;
+    memset(coded_out, 0, (size_t)coded_bits);
+    const int m = n - k;
+    for (int b = 0; b < blocks; ++b) {
+        const uint8_t* in = info + (size_t)b * k;
+        uint8_t* out = coded_out + (size_t)b * n;
+        if (dvb_s2) {
+            for (int i = 0; i < k; ++i) out[i] = in[i] & 1u;
+            uint8_t prev = 0;
+            for (int r = 0; r < m; ++r) {
+                uint8_t s = 0;
+                for (int j = row_info_offsets[r]; j < row_info_offsets[r + 1]; ++j) s ^= in[row_info_cols[j]] & 1u;
+                const uint8_t p = (uint8_t)(s ^ (r == 0 ? 0u : prev));
+                out[k + r] = p;
+                prev = p;
+            }
+        } else {
+            for (int c = 0; c < n; ++c) {
+                uint8_t acc = 0;
+                for (int r = 0; r < k; ++r) acc ^= (uint8_t)((in[r] & 1u) & (gen_kn[(size_t)r * n + c] & 1u));
+                out[c] = acc;
+            }
+        }
+    }
+    const uint32_t seed32 = (uint32_t)(seed ^ (seed >> 32));
+    const float sigma = sqrtf((noise_var > 0.0f ? noise_var : 0.0f) * 0.5f);
+    const float scale = amplitude / (float)nfft;
+    double* re = (double*)malloc(sizeof(double) * (size_t)nfft);
+    double* im = (double*)malloc(sizeof(double) * (size_t)nfft);
+    float* g = (float*)malloc(sizeof(float) * 2 * (size_t)nfft);
+    for (int i = 0; i < pre_len; ++i) {
+        frame_out[2 * i] = preamble[2 * i] * amplitude;
+        frame_out[2 * i + 1] = preamble[2 * i + 1] * amplitude;
+    }
+    for (int sym = 0; sym < num_symbols; ++sym) {
+        for (int i = 0; i < 2 * nfft; ++i) g[i] = 0.0f;
+        const int pi = pilot_index[sym], di = data_index[sym];
+        for (int sc = 0; sc < active_sc; ++sc) {
+            const int re_idx = sym * active_sc + sc;
+            float vr = 0.0f, vi = 0.0f;
+            if (pi >= 0) {
+                vr = pilot_grid[2 * (pi * active_sc + sc)];
+                vi = pilot_grid[2 * (pi * active_sc + sc) + 1];
+            } else if (di >= 0) {
+                const int base = (di * active_sc + sc) * bits_per_symbol;
+                int idx = 0;
+                for (int i = 0; i < bits_per_symbol; ++i)
+                    idx = (idx << 1) | (base + i < coded_bits ? (coded_out[base + i] & 1) : 0);
+                vr = constellation[2 * idx];
+                vi = constellation[2 * idx + 1];
+            }
+            if (precomp_h) {
+                const float hr = precomp_h[2 * re_idx], hi = precomp_h[2 * re_idx + 1];
+                float h2 = normf(hr, hi);
+                if (h2 < 1.0e-10f) h2 = 1.0e-10f;
+                const float inv = 1.0f / h2;
+                const float cr = hr * inv, ci = -hi * inv;
+                const float tr = vr * cr - vi * ci, ti = vr * ci + vi * cr;
+                vr = tr;
+                vi = ti;
+            }
+            if (actual_h) {
+                const float hr = actual_h[2 * re_idx], hi = actual_h[2 * re_idx + 1];
+                const float tr = vr * hr - vi * hi, ti = vr * hi + vi * hr;
+                vr = tr;
+                vi = ti;
+            }
+            if (noise_var > 0.0f && di >= 0) {
+                vr += sigma * tx_normal(seed32, (uint32_t)re_idx);
+                vi += sigma * tx_normal(seed32 ^ 0xa5a5a5a5u, (uint32_t)re_idx);
+            }
+            const int bin = (used_shifted[sc] + nfft / 2) % nfft;
+            g[2 * bin] = vr;
+            g[2 * bin + 1] = vi;
+        }
+        for (int i = 0; i < nfft; ++i) {
+            re[i] = g[2 * i];
+            im[i] = g[2 * i + 1];
+        }
+        oracle_fft(re, im, nfft, 1); /* 1/n normalized; undo for cuFFT's unnormalized inverse */
+        float* o = frame_out + 2 * ((size_t)pre_len + (size_t)sym * (nfft + cp));
+        for (int i = 0; i < nfft + cp; ++i) {
+            const int src = i < cp ? nfft - cp + i : i - cp;
+            o[2 * i] = (float)(re[src] * nfft) * scale;
+            o[2 * i + 1] = (float)(im[src] * nfft) * scale;
+        }
+    }
+    free(re);
+    free(im);
+    free(g);
+    return 0;
+}

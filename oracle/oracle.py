@@ -183,3 +183,52 @@ def demod_frame(frame: np.ndarray, phy, cfo_hz: float, noise_var: float) -> np.n
                               phy.bits_per_symbol, _p(const, f32p), C.c_double(cfo_hz), C.c_double(phy.fs),
                               C.c_double(noise_var), _p(out, f32p))
     return out
+
+
+def symbol_index_tables(num_symbols: int, pilot_symbols, data_symbols):
+    """Per-symbol pilot / data ordinals (-1 where the symbol is of the other kind)."""
+    pi = np.full(num_symbols, -1, np.int32)
+    di = np.full(num_symbols, -1, np.int32)
+    for i, s in enumerate(pilot_symbols):
+        pi[s] = i
+    for i, s in enumerate(data_symbols):
+        di[s] = i
+    return pi, di
+
+
+def tx_frame(info: np.ndarray, n: int, k: int, phy, amplitude: float, gen_kn: np.ndarray | None = None,
+             dvb_rows: tuple | None = None, coded_bits: int | None = None, noise_var: float = 0.0, seed: int = 0,
+             precomp_h: np.ndarray | None = None, actual_h: np.ndarray | None = None):
+    """PIPE:129-342 transmitter on one frame -> (coded bits, complex64 frame).
+
+    info: [blocks * k] bytes; gen_kn [k][n] bytes, or dvb_rows = (row_info_offsets,
+    row_info_cols) for the DVB-S2 accumulator encoder."""
+    info = np.ascontiguousarray(info, np.uint8).reshape(-1)
+    blocks = info.size // k
+    if coded_bits is None:
+        coded_bits = len(phy.data_symbols) * phy.active_sc * phy.bits_per_symbol
+    pi, di = symbol_index_tables(phy.num_symbols, phy.pilot_symbols, phy.data_symbols)
+    used = np.ascontiguousarray(phy.used_shifted, np.int32)
+    pg = _iq(phy.pilot_grid.reshape(-1))
+    pre = _iq(phy.preamble)
+    const = _iq(phy.constellation)
+    gen = None if gen_kn is None else np.ascontiguousarray(gen_kn, np.uint8)
+    ro = rc = None
+    if dvb_rows is not None:
+        ro = np.ascontiguousarray(dvb_rows[0], np.int32)
+        rc = np.ascontiguousarray(dvb_rows[1], np.int32)
+        if rc.size == 0:
+            rc = np.zeros(1, np.int32)
+    pc = None if precomp_h is None else _iq(np.asarray(precomp_h).reshape(-1))
+    ac = None if actual_h is None else _iq(np.asarray(actual_h).reshape(-1))
+    frame_len = phy.pre_len + phy.num_symbols * (phy.nfft + phy.cp)
+    coded = np.zeros(coded_bits, np.uint8)
+    out = np.zeros(2 * frame_len, np.float32)
+    r = load().oracle_tx_frame(
+        _p(info, u8p), n, k, blocks, _p(gen, u8p), int(dvb_rows is not None), _p(ro, i32p), _p(rc, i32p),
+        coded_bits, phy.nfft, phy.cp, phy.pre_len, phy.num_symbols, phy.active_sc, phy.bits_per_symbol,
+        _p(pi, i32p), _p(di, i32p), _p(used, i32p), _p(pg, f32p), _p(pre, f32p), _p(const, f32p), _p(pc, f32p),
+        _p(ac, f32p), C.c_float(amplitude), C.c_float(noise_var), C.c_uint64(seed), _p(coded, u8p), _p(out, f32p))
+    if r != 0:
+        raise ValueError("oracle_tx_frame: bad shape")
+    return coded, out.view(np.complex64)

@@ -75,6 +75,21 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
                         int bits_per_symbol, const float* constellation, double cfo_hz, double fs,
                         double noise_var, float* llr_out);
 
+/* ---- OFDM transmit, GPU transmitter semantics (PIPE:129-342) ----
+ * info [blocks][k] bytes; gen_kn [k][n] bytes (generator path) or, when
+ * dvb_s2, row_info_offsets [n-k+1] / row_info_cols (accumulator path).
+ * coded_out [coded_bits] (pad bits beyond blocks * n are 0); frame_out
+ * [pre_len + num_symbols * (nfft + cp)] interleaved IQ.  precomp_h / actual_h
+ * are [num_symbols][active_sc] IQ or NULL; noise_var > 0 adds the counter-based
+ * AWGN of seed on data resource elements.  Returns 0, or -1 on a bad shape. */
+int oracle_tx_frame(const uint8_t* info, int n, int k, int blocks, const uint8_t* gen_kn, int dvb_s2,
+                    const int* row_info_offsets, const int* row_info_cols, int coded_bits, int nfft, int cp,
+                    int pre_len, int num_symbols, int active_sc, int bits_per_symbol, const int* pilot_index,
+                    const int* data_index, const int* used_shifted, const float* pilot_grid,
+                    const float* preamble, const float* constellation, const float* precomp_h,
+                    const float* actual_h, float amplitude, float noise_var, uint64_t seed,
+                    uint8_t* coded_out, float* frame_out);
+
 #ifdef __cplusplus
 }
 #endif

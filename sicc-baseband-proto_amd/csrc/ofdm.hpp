@@ -34,6 +34,7 @@ struct OfdmSyncArgs {
 struct OfdmDemodArgs {
     const float2* rx = nullptr;  // [frames][span], frame f starts at sync[f].start
     int span = 0;
+    int frame_samples = 0;  // preamble_len + num_symbols * (nfft + cp)
     const OfdmSyncResult* sync = nullptr;
     int nfft = 1024;
     int cp = 72;
@@ -62,5 +63,46 @@ struct OfdmDemodArgs {
 
 void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s);
 void launch_ofdm_demod(const OfdmDemodArgs& a, int frames, hipStream_t s);
+
+// Transmit side (the reference GPU transmitter, gpu_ofdm_pipeline.cu:129-342).
+struct OfdmTxArgs {
+    // encoder: info [frames][blocks][k] bytes -> coded [frames][coded_bits] bytes
+    const uint8_t* info = nullptr;
+    int code_n = 0;
+    int code_k = 0;
+    int blocks = 0;
+    // generator encoder: bit r of gen_cols[c * gen_words + r / 64] is G[r][c]
+    const unsigned long long* gen_cols = nullptr;
+    int gen_words = 0;
+    // DVB-S2 accumulator encoder (used when dvb_s2 != 0): systematic info in
+    // bits [0, k), parity p_r = rowsum_r ^ p_{r-1} in bits k + r, where rowsum_r
+    // is the XOR of info[row_info_cols[j]] for j in [row_info_offsets[r], row_info_offsets[r + 1]).
+    int dvb_s2 = 0;
+    const int* row_info_offsets = nullptr;  // [n - k + 1]
+    const int* row_info_cols = nullptr;
+    uint8_t* coded = nullptr;               // [frames][coded_bits]
+    int coded_bits = 0;                     // data capacity in bits (pad bits are 0)
+    // grid + IFFT + compose
+    int nfft = 1024;
+    int cp = 72;
+    int preamble_len = 1024;
+    int num_symbols = 14;
+    int active_sc = 600;
+    int bits_per_symbol = 4;
+    const int* pilot_index = nullptr;     // [num_symbols] pilot ordinal or -1
+    const int* data_index = nullptr;      // [num_symbols] data ordinal or -1
+    const int* used_natural = nullptr;    // [active_sc]
+    const float2* pilot_grid = nullptr;   // [n_pilot][active_sc]
+    const float2* preamble = nullptr;     // [preamble_len]
+    const float2* constellation = nullptr;
+    const float2* precomp_h = nullptr;    // optional [num_symbols][active_sc]
+    const float2* actual_h = nullptr;     // optional [num_symbols][active_sc]
+    const double2* twiddle = nullptr;     // [nfft] exp(-2 pi i m / nfft)
+    float amplitude = 0.2f;
+    float noise_var = 0.0f;               // frequency-domain AWGN variance on data REs (0: none)
+    unsigned long long noise_seed = 0;    // frame f of a batch uses noise_seed + f
+    float2* out = nullptr;                // [frames][frame_samples]
+};
+void launch_ofdm_tx(const OfdmTxArgs& a, int frames, hipStream_t s);
 
 }  // namespace sicc

@@ -40,22 +40,39 @@ std::vector<float2> qam_constellation(int bits)
     return c;
 }
 
-OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted, int n_pilot,
-               const int* pilot_symbols, const float2* pilot_grid)
-    : cfg_(cfg)
+namespace {
+
+void check_common(const sicc_ofdm_config_t& cfg)
 {
     if (cfg.nfft <= 0 || (cfg.nfft & (cfg.nfft - 1)) != 0) throw std::invalid_argument("nfft must be a power of two");
     if (cfg.cp < 0 || cfg.cp >= cfg.nfft) throw std::invalid_argument("cp must be in [0, nfft)");
     if (cfg.pre_half_len <= 0 || cfg.preamble_len != 2 * cfg.pre_half_len)
         throw std::invalid_argument("preamble_len must be 2 * pre_half_len (two identical halves)");
     if (cfg.active_sc <= 0 || cfg.active_sc >= cfg.nfft) throw std::invalid_argument("active_sc must be in (0, nfft)");
-    if (cfg.num_symbols <= 0 || n_pilot < 0 || n_pilot >= cfg.num_symbols)
-        throw std::invalid_argument("need 0 <= pilots < num_symbols");
+    if (cfg.num_symbols <= 0) throw std::invalid_argument("num_symbols must be positive");
+}
+
+std::vector<int> natural_bins(const sicc_ofdm_config_t& cfg, const int* used_shifted)
+{
+    std::vector<int> nat(cfg.active_sc);
+    for (int sc = 0; sc < cfg.active_sc; ++sc) {
+        if (used_shifted[sc] < 0 || used_shifted[sc] >= cfg.nfft) throw std::invalid_argument("subcarrier out of range");
+        nat[sc] = (used_shifted[sc] + cfg.nfft / 2) % cfg.nfft;
+    }
+    return nat;
+}
+
+}  // namespace
+
+OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted, int n_pilot,
+               const int* pilot_symbols, const float2* pilot_grid)
+    : cfg_(cfg)
+{
+    check_common(cfg);
+    if (n_pilot < 0 || n_pilot >= cfg.num_symbols) throw std::invalid_argument("need 0 <= pilots < num_symbols");
     if (cfg.use_pilot_channel_est && n_pilot == 0) throw std::invalid_argument("pilot estimation needs a pilot symbol");
     if (!preamble || !used_shifted || (n_pilot > 0 && (!pilot_symbols || !pilot_grid)))
         throw std::invalid_argument("null table");
-    if (cfg_.refine_radius < 0) cfg_.refine_radius = std::min(16, cfg.cp);
-    n_pilot_ = n_pilot;
     std::vector<int> ps(pilot_symbols, pilot_symbols + n_pilot);
     for (size_t i = 0; i < ps.size(); ++i) {
         if (ps[i] < 0 || ps[i] >= cfg.num_symbols) throw std::invalid_argument("pilot symbol out of range");
@@ -64,14 +81,11 @@ OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int*
     std::vector<int> ds;
     for (int s = 0; s < cfg.num_symbols; ++s)
         if (std::find(ps.begin(), ps.end(), s) == ps.end()) ds.push_back(s);
-    n_data_ = static_cast<int>(ds.size());
-    const long capacity = static_cast<long>(n_data_) * cfg.active_sc * cfg.bits_per_symbol;
-    if (cfg.blocks_per_frame <= 0 || cfg.code_n <= 0 || static_cast<long>(cfg.blocks_per_frame) * cfg.code_n > capacity)
-        throw std::invalid_argument("blocks_per_frame * code_n must be positive and fit the data resource elements");
+    const int n_data = static_cast<int>(ds.size());
     // interpolation tables (decode_frame, uhd_ldpc_ofdm_link.cpp:5402-5420)
-    std::vector<int> p0(n_data_, 0), p1(n_data_, 0);
-    std::vector<float> w0(n_data_, 1.0f), w1(n_data_, 0.0f);
-    for (int di = 0; di < n_data_ && n_pilot > 0; ++di) {
+    std::vector<int> p0(n_data, 0), p1(n_data, 0);
+    std::vector<float> w0(n_data, 1.0f), w1(n_data, 0.0f);
+    for (int di = 0; di < n_data && n_pilot > 0; ++di) {
         const int sym = ds[di];
         int hi = 0;
         while (hi + 1 < n_pilot && sym > ps[hi + 1]) ++hi;
@@ -85,11 +99,49 @@ OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int*
         w0[di] = static_cast<float>(1.0 - alpha);
         w1[di] = static_cast<float>(alpha);
     }
-    std::vector<int> nat(cfg.active_sc);
-    for (int sc = 0; sc < cfg.active_sc; ++sc) {
-        if (used_shifted[sc] < 0 || used_shifted[sc] >= cfg.nfft) throw std::invalid_argument("subcarrier out of range");
-        nat[sc] = (used_shifted[sc] + cfg.nfft / 2) % cfg.nfft;
+    init(preamble, natural_bins(cfg, used_shifted), ps, pilot_grid, ds, p0, p1, w0, w1);
+}
+
+OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted,
+               const std::vector<int>& pilot_symbols, const float2* pilot_grid, const std::vector<int>& data_symbols,
+               const std::vector<int>& interp_p0, const std::vector<int>& interp_p1,
+               const std::vector<float>& interp_alpha)
+    : cfg_(cfg)
+{
+    check_common(cfg);
+    const int n_pilot = static_cast<int>(pilot_symbols.size());
+    const size_t nd = data_symbols.size();
+    if (!preamble || !used_shifted || (n_pilot > 0 && !pilot_grid)) throw std::invalid_argument("null table");
+    if (interp_p0.size() != nd || interp_p1.size() != nd || interp_alpha.size() != nd)
+        throw std::invalid_argument("interpolation tables must have one entry per data symbol");
+    if (cfg.use_pilot_channel_est && n_pilot == 0) throw std::invalid_argument("pilot estimation needs a pilot symbol");
+    for (int s : pilot_symbols)
+        if (s < 0 || s >= cfg.num_symbols) throw std::invalid_argument("pilot symbol out of range");
+    for (int s : data_symbols)
+        if (s < 0 || s >= cfg.num_symbols) throw std::invalid_argument("data symbol out of range");
+    std::vector<float> w0(nd), w1(nd);
+    for (size_t i = 0; i < nd; ++i) {
+        if (cfg.use_pilot_channel_est && (interp_p0[i] < 0 || interp_p0[i] >= n_pilot || interp_p1[i] < 0 ||
+                                          interp_p1[i] >= n_pilot))
+            throw std::invalid_argument("interpolation pilot index out of range");
+        w0[i] = 1.0f - interp_alpha[i];
+        w1[i] = interp_alpha[i];
     }
+    init(preamble, natural_bins(cfg, used_shifted), pilot_symbols, pilot_grid, data_symbols, interp_p0, interp_p1, w0,
+         w1);
+}
+
+void OfdmRx::init(const float2* preamble, const std::vector<int>& nat, const std::vector<int>& ps,
+                  const float2* pilot_grid, const std::vector<int>& ds, const std::vector<int>& p0,
+                  const std::vector<int>& p1, const std::vector<float>& w0, const std::vector<float>& w1)
+{
+    const auto& cfg = cfg_;
+    if (cfg_.refine_radius < 0) cfg_.refine_radius = std::min(16, cfg.cp);
+    n_pilot_ = static_cast<int>(ps.size());
+    n_data_ = static_cast<int>(ds.size());
+    const long capacity = static_cast<long>(n_data_) * cfg.active_sc * cfg.bits_per_symbol;
+    if (cfg.blocks_per_frame <= 0 || cfg.code_n <= 0 || static_cast<long>(cfg.blocks_per_frame) * cfg.code_n > capacity)
+        throw std::invalid_argument("blocks_per_frame * code_n must be positive and fit the data resource elements");
     pre_energy_ = 0.0;
     for (int i = 0; i < cfg.preamble_len; ++i)
         pre_energy_ += static_cast<double>(preamble[i].x * preamble[i].x + preamble[i].y * preamble[i].y);
@@ -105,7 +157,7 @@ OfdmRx::OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int*
         if (count) SICC_HIP(hipMemcpy(buf.get(), src, count * sizeof(*src), hipMemcpyHostToDevice));
     };
     up(preamble_, preamble, cfg.preamble_len);
-    up(pilot_grid_, pilot_grid, static_cast<size_t>(n_pilot) * cfg.active_sc);
+    up(pilot_grid_, pilot_grid, static_cast<size_t>(n_pilot_) * cfg.active_sc);
     up(constellation_, con.data(), con.size());
     up(pilot_symbols_, ps.data(), ps.size());
     up(data_symbols_, ds.data(), ds.size());
@@ -143,6 +195,7 @@ void OfdmRx::demod(const float2* d_rx, int frames, int span, const OfdmSyncResul
     OfdmDemodArgs a;
     a.rx = d_rx;
     a.span = span;
+    a.frame_samples = frame_samples();
     a.sync = d_sync;
     a.nfft = cfg_.nfft;
     a.cp = cfg_.cp;

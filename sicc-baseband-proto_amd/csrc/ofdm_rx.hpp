@@ -15,6 +15,13 @@ public:
     // pilot_symbols [n_pilot], pilot_grid [n_pilot][active_sc] complex.
     OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted, int n_pilot,
            const int* pilot_symbols, const float2* pilot_grid);
+    // Caller-supplied symbol and interpolation tables (the pipeline front's
+    // StaticTables): data symbol d is equalized with
+    // (1 - alpha[d]) * h[p0[d]] + alpha[d] * h[p1[d]] (float weights).
+    OfdmRx(const sicc_ofdm_config_t& cfg, const float2* preamble, const int* used_shifted,
+           const std::vector<int>& pilot_symbols, const float2* pilot_grid, const std::vector<int>& data_symbols,
+           const std::vector<int>& interp_p0, const std::vector<int>& interp_p1,
+           const std::vector<float>& interp_alpha);
 
     const sicc_ofdm_config_t& config() const { return cfg_; }
     int frame_samples() const { return cfg_.preamble_len + cfg_.num_symbols * (cfg_.nfft + cfg_.cp); }
@@ -24,7 +31,18 @@ public:
     void demod(const float2* d_rx, int frames, int span, const OfdmSyncResult* d_sync, float* d_llr,
                hipStream_t s) const;
 
+    // device tables shared with a transmitter of the same numerology
+    const float2* d_preamble() const { return preamble_.get(); }
+    const float2* d_pilot_grid() const { return pilot_grid_.get(); }
+    const float2* d_constellation() const { return constellation_.get(); }
+    const int* d_used_natural() const { return used_natural_.get(); }
+    const double2* d_twiddle() const { return twiddle_.get(); }
+
 private:
+    void init(const float2* preamble, const std::vector<int>& nat, const std::vector<int>& ps,
+              const float2* pilot_grid, const std::vector<int>& ds, const std::vector<int>& p0,
+              const std::vector<int>& p1, const std::vector<float>& w0, const std::vector<float>& w1);
+
     sicc_ofdm_config_t cfg_;
     int n_pilot_ = 0;
     int n_data_ = 0;

@@ -73,6 +73,12 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    # torch ships its own HIP and HSA runtimes.  Mapping them first lets the
+    # library's libamdhip64.so.7 / libhsa-runtime64.so.1 dependencies resolve to
+    # those same copies; loaded the other way round the process would hold two
+    # HSA runtimes and torch would find no device.
+    import torch  # noqa: F401
+
     lib = C.CDLL(LIB_PATH)
     lib.cuda_osd_get_last_error.restype = C.c_char_p
     lib.cuda_osd_get_decoded_codewords.restype = C.c_ulonglong
@@ -530,3 +536,199 @@ def qc_code(n: int, mb: int, col_weight: int = 3, seed: int = 1):
     ci = np.zeros(e.value, np.int32)
     check(lib.sicc_qc_code_csr(n, mb, col_weight, seed, None, None, ptr(rp, i32p), ptr(ci, i32p)))
     return m.value, rp, ci
+
+
+# ------------------------------------------------ C++ pipeline front ----
+class PipelineConfig(C.Structure):
+    _fields_ = [
+        ("nfft", C.c_int), ("cp", C.c_int), ("preamble_len", C.c_int), ("pre_half_len", C.c_int),
+        ("num_symbols", C.c_int), ("active_sc", C.c_int), ("bits_per_symbol", C.c_int), ("code_n", C.c_int),
+        ("code_k", C.c_int), ("blocks_per_frame", C.c_int), ("max_rx_search_samples", C.c_int),
+        ("sample_rate", C.c_float), ("sync_threshold", C.c_float), ("tx_amplitude", C.c_float),
+        ("demod_noise_var", C.c_float), ("use_pilot_channel_est", C.c_int),
+    ]
+
+
+_TABLE_FIELDS = [
+    ("generator_kn", C.c_void_p), ("dvb_s2_accumulator", None), ("row_info_offsets", C.c_void_p),
+    ("row_info_cols", C.c_void_p), ("used_indices", C.c_void_p), ("pilot_symbol_index", C.c_void_p),
+    ("data_symbol_index", C.c_void_p), ("pilot_symbols", C.c_void_p), ("data_symbols", C.c_void_p),
+    ("interp_p0", C.c_void_p), ("interp_p1", C.c_void_p), ("interp_alpha", C.c_void_p),
+    ("pilot_grid_iq", C.c_void_p), ("preamble_iq", C.c_void_p), ("precomp_h_iq", C.c_void_p),
+    ("actual_h_iq", C.c_void_p),
+]
+
+
+def _table_struct_fields():
+    out = []
+    for name, t in _TABLE_FIELDS:
+        if t is None:
+            out.append((name, C.c_int))
+        else:
+            out.append((name, t))
+            out.append((name.replace("_iq", "") + "_count", C.c_int))
+    return out
+
+
+class PipelineTables(C.Structure):
+    _fields_ = _table_struct_fields()
+
+
+class PipelineSync(C.Structure):
+    _fields_ = [("start", C.c_int), ("metric", C.c_float), ("cfo_hz", C.c_float)]
+
+    def astuple(self):
+        return self.start, self.metric, self.cfo_hz
+
+
+def device_to_host(ptr: int, nbytes: int) -> bytes:
+    """Synchronous copy out of device memory (the HIP runtime the library links)."""
+    lib = load()
+    buf = C.create_string_buffer(nbytes)
+    lib.hipDeviceSynchronize()
+    rc = lib.hipMemcpy(buf, C.c_void_p(ptr), C.c_size_t(nbytes), 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    return buf.raw
+
+
+class Pipeline:
+    """usrp_gpu_pipeline_bridge::Pipeline through its C layer (sicc_pipeline_*)."""
+
+    def __init__(self, phy, code_n: int, code_k: int, blocks_per_frame: int, slots: int = 3,
+                 max_rx_search_samples: int = 0, tx_amplitude: float = 0.2, noise_var: float = 1e-3,
+                 threshold: float = 0.45, use_pilots: bool = True):
+        self.lib = lib = load()
+        lib.sicc_pipeline_create.argtypes = [C.POINTER(PipelineConfig), C.c_int, C.POINTER(C.c_ulonglong)]
+        lib.sicc_pipeline_destroy.argtypes = [C.c_ulonglong]
+        lib.sicc_pipeline_upload_static_tables.argtypes = [C.c_ulonglong, C.POINTER(PipelineTables)]
+        lib.sicc_pipeline_submit_tx_frame.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p, C.c_void_p]
+        lib.sicc_pipeline_submit_tx_frame_device.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p, C.c_float,
+                                                             C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]
+        lib.sicc_pipeline_synchronize_tx.argtypes = [C.c_ulonglong, C.c_int]
+        lib.sicc_pipeline_find_frame.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p, C.c_int,
+                                                 C.POINTER(PipelineSync)]
+        lib.sicc_pipeline_find_frame_device.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p, C.c_int,
+                                                        C.POINTER(PipelineSync)]
+        lib.sicc_pipeline_demod_frame_to_device_llr.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p,
+                                                                C.POINTER(PipelineSync), C.POINTER(C.c_void_p)]
+        lib.sicc_pipeline_demod_frame_device_to_device_llr.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p,
+                                                                       C.POINTER(PipelineSync),
+                                                                       C.POINTER(C.c_void_p)]
+        lib.sicc_pipeline_demod_frame_to_host_llr.argtypes = [C.c_ulonglong, C.c_int, C.c_void_p, C.c_void_p,
+                                                              C.POINTER(PipelineSync)]
+        lib.sicc_pipeline_tx_batch_device.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, C.c_float,
+                                                      C.c_ulonglong, C.c_void_p, C.c_void_p]
+        lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.phy = phy
+        self.cfg = PipelineConfig(
+            nfft=phy.nfft, cp=phy.cp, preamble_len=phy.pre_len, pre_half_len=phy.pre_half_len,
+            num_symbols=phy.num_symbols, active_sc=phy.active_sc, bits_per_symbol=phy.bits_per_symbol,
+            code_n=code_n, code_k=code_k, blocks_per_frame=blocks_per_frame,
+            max_rx_search_samples=max_rx_search_samples, sample_rate=phy.fs, sync_threshold=threshold,
+            tx_amplitude=tx_amplitude, demod_noise_var=noise_var, use_pilot_channel_est=int(use_pilots))
+        self.frame_samples = phy.pre_len + phy.num_symbols * (phy.nfft + phy.cp)
+        self.coded_bits = blocks_per_frame * code_n
+        self.info_bits = blocks_per_frame * code_k
+        h = C.c_ulonglong(0)
+        check(lib.sicc_pipeline_create(C.byref(self.cfg), slots, C.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            self.lib.sicc_pipeline_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_static_tables(self, **t):
+        """Keyword arguments named as StaticTables' fields (numpy arrays or None)."""
+        conv = {
+            "generator_kn": np.uint8, "row_info_offsets": np.int32, "row_info_cols": np.int32,
+            "used_indices": np.int32, "pilot_symbol_index": np.int32, "data_symbol_index": np.int32,
+            "pilot_symbols": np.int32, "data_symbols": np.int32, "interp_p0": np.int32, "interp_p1": np.int32,
+            "interp_alpha": np.float32, "pilot_grid": np.complex64, "preamble": np.complex64,
+            "precomp_h": np.complex64, "actual_h": np.complex64,
+        }
+        st = PipelineTables()
+        self._keep = []
+        for name, dt in conv.items():
+            v = t.get(name)
+            field = name + "_iq" if dt is np.complex64 else name
+            if v is None:
+                continue
+            a = np.ascontiguousarray(np.asarray(v, dt).reshape(-1))
+            self._keep.append(a)
+            setattr(st, field, a.ctypes.data)
+            setattr(st, name + "_count", a.size)
+        st.dvb_s2_accumulator = int(bool(t.get("dvb_s2_accumulator", False)))
+        check(self.lib.sicc_pipeline_upload_static_tables(self.handle, C.byref(st)))
+
+    def submit_tx_frame(self, slot: int, info: np.ndarray) -> np.ndarray:
+        info = np.ascontiguousarray(info, np.uint8).reshape(-1)
+        out = np.zeros(self.frame_samples, np.complex64)
+        check(self.lib.sicc_pipeline_submit_tx_frame(self.handle, slot, info.ctypes.data,
+                                                                out.ctypes.data))
+        check(self.lib.sicc_pipeline_synchronize_tx(self.handle, slot))
+        return out
+
+    def submit_tx_frame_device(self, slot: int, info: np.ndarray, noise_var: float = 0.0, seed: int = 0,
+                               add_noise: bool = False) -> int:
+        info = np.ascontiguousarray(info, np.uint8).reshape(-1)
+        p = C.c_void_p()
+        check(self.lib.sicc_pipeline_submit_tx_frame_device(self.handle, slot, info.ctypes.data,
+                                                                       noise_var, seed, int(add_noise),
+                                                                       C.byref(p)))
+        return p.value
+
+    def synchronize_tx(self, slot: int):
+        check(self.lib.sicc_pipeline_synchronize_tx(self.handle, slot))
+
+    def read_frame(self, d_ptr: int) -> np.ndarray:
+        return np.frombuffer(device_to_host(d_ptr, 8 * self.frame_samples), np.complex64).copy()
+
+    def read_llr(self, d_ptr: int) -> np.ndarray:
+        return np.frombuffer(device_to_host(d_ptr, 4 * self.coded_bits), np.float32).copy()
+
+    def find_frame(self, slot: int, samples: np.ndarray):
+        s = np.ascontiguousarray(samples, np.complex64)
+        r = PipelineSync()
+        check(self.lib.sicc_pipeline_find_frame(self.handle, slot, s.ctypes.data, s.size, C.byref(r)))
+        return r.astuple()
+
+    def find_frame_device(self, slot: int, d_ptr: int, count: int):
+        r = PipelineSync()
+        check(self.lib.sicc_pipeline_find_frame_device(self.handle, slot, d_ptr, count, C.byref(r)))
+        return r.astuple()
+
+    def demod_frame_to_device_llr(self, slot: int, frame: np.ndarray):
+        f = np.ascontiguousarray(frame, np.complex64)
+        r = PipelineSync()
+        p = C.c_void_p()
+        check(self.lib.sicc_pipeline_demod_frame_to_device_llr(self.handle, slot, f.ctypes.data,
+                                                                          C.byref(r), C.byref(p)))
+        return p.value, r.astuple()
+
+    def demod_frame_device_to_device_llr(self, slot: int, d_frame: int, sync) -> int:
+        r = PipelineSync(*sync)
+        p = C.c_void_p()
+        check(self.lib.sicc_pipeline_demod_frame_device_to_device_llr(self.handle, slot, d_frame,
+                                                                                 C.byref(r), C.byref(p)))
+        return p.value
+
+    def demod_frame_to_host_llr(self, slot: int, frame: np.ndarray):
+        f = np.ascontiguousarray(frame, np.complex64)
+        llr = np.zeros(self.coded_bits, np.float32)
+        r = PipelineSync()
+        check(self.lib.sicc_pipeline_demod_frame_to_host_llr(self.handle, slot, f.ctypes.data,
+                                                                        llr.ctypes.data, C.byref(r)))
+        return llr, r.astuple()
+
+    def tx_batch_device(self, d_info: int, frames: int, d_out: int, noise_var: float = 0.0, seed: int = 0,
+                        stream: int = 0):
+        check(self.lib.sicc_pipeline_tx_batch_device(self.handle, d_info, frames, noise_var, seed,
+                                                                d_out, stream or None))

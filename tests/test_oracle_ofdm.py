@@ -179,3 +179,134 @@ def test_noise_free_loopback_known_answer():
         frame = wf.tx_frames(phy, coded)[0]
         llr = oracle.demod_frame(frame, phy, 0.0, 1e-6)
         assert np.array_equal((llr < 0).astype(np.uint8), coded[0])
+
+
+# ------------------------------------------------------------- transmit ----
+@needs_ref
+@pytest.mark.parametrize("mod", ["qpsk", "16qam"])
+def test_tx_matches_reference_build_tx_frame(mod):
+    # The GPU transmitter's frame (PIPE:129-342: preamble * amp, body = IFFT * amp / N)
+    # is the reference CPU transmitter's frame (LINK:4590-4629, one peak scale for the
+    # whole frame) times one real constant: same encoder, grid, IFFT, CP and layout.
+    phy = refbind.RefPhy(active_sc=512, modulation=mod, pilot_period=7)
+    rcode = refbind.RefCode(wf_alist(), True)
+    n_, m_, rp, ci = oracle.alist_parse(wf_alist())
+    ocode = oracle.Code(n_, m_, rp, ci, True)
+    gen = ocode.generator()
+    rng = np.random.default_rng(11)
+    blocks = phy.coded_bits // 128
+    for _ in range(2):
+        info = rng.integers(0, 2, blocks * 64, dtype=np.uint8)
+        coded, ours = oracle.tx_frame(info, 128, 64, phy, amplitude=0.2, gen_kn=gen)
+        assert np.array_equal(coded, rcode.encode(info.reshape(blocks, 64)).reshape(-1))
+        ref = phy.build_tx_frame(info).astype(np.complex128)
+        s = np.vdot(ref, ours) / np.vdot(ref, ref)
+        assert abs(s.imag) < 1e-6 * abs(s.real) and s.real > 0
+        err = np.abs(ours - s.real * ref).max()
+        assert err <= 2e-6 * np.abs(ours).max(), err
+
+
+def _grid_of(frame, phy, amp):
+    """Recover each symbol's grid at the used bins from a transmitted frame (numpy FFT)."""
+    out = np.zeros((phy.num_symbols, phy.active_sc), np.complex128)
+    bins = (np.asarray(phy.used_shifted) + phy.nfft // 2) % phy.nfft
+    for s in range(phy.num_symbols):
+        b = phy.pre_len + s * (phy.nfft + phy.cp) + phy.cp
+        out[s] = np.fft.fft(frame[b:b + phy.nfft].astype(np.complex128))[bins] / amp
+    return out
+
+
+def _wf_phy_and_code():
+    phy = wf.OfdmPhy(bits_per_symbol=2, active_sc=512, num_symbols=6, pilot_symbols=[0, 3])
+    n_, m_, rp, ci = oracle.alist_parse(wf_alist())
+    return phy, oracle.Code(n_, m_, rp, ci, True)
+
+
+def test_tx_grid_cp_and_channel_closed_forms():
+    phy, code = _wf_phy_and_code()
+    gen = code.generator()
+    rng = np.random.default_rng(12)
+    blocks = phy.coded_bits // 128
+    info = rng.integers(0, 2, blocks * 64, dtype=np.uint8)
+    amp = 0.3
+    coded, frame = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen)
+    # preamble scaled by amp; cyclic prefix = tail of each symbol
+    assert np.allclose(frame[:phy.pre_len], phy.preamble * np.float32(amp), rtol=0, atol=1e-7)
+    for s in range(phy.num_symbols):
+        b = phy.pre_len + s * (phy.nfft + phy.cp)
+        assert np.array_equal(frame[b:b + phy.cp], frame[b + phy.nfft:b + phy.nfft + phy.cp])
+    # FFT of each body gives back pilots and Gray-mapped data (MSB first)
+    g = _grid_of(frame, phy, amp)
+    for i, s in enumerate(phy.pilot_symbols):
+        assert np.allclose(g[s], phy.pilot_grid[i], atol=1e-5)
+    B = phy.bits_per_symbol
+    for i, s in enumerate(phy.data_symbols):
+        bits = coded[i * phy.active_sc * B:(i + 1) * phy.active_sc * B].reshape(phy.active_sc, B)
+        idx = (bits * (1 << np.arange(B - 1, -1, -1))).sum(1)
+        assert np.allclose(g[s], phy.constellation[idx], atol=1e-5)
+    # actual channel multiplies every resource element; pre-equalization by the
+    # same channel undoes it (v conj(h) / |h|^2 * h = v)
+    h = (rng.standard_normal((phy.num_symbols, phy.active_sc)) +
+         1j * rng.standard_normal((phy.num_symbols, phy.active_sc))).astype(np.complex64)
+    _, fa = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, actual_h=h)
+    assert np.allclose(_grid_of(fa, phy, amp), g * h, atol=1e-4)
+    _, fb = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, actual_h=h, precomp_h=h)
+    assert np.allclose(fb, frame, atol=1e-5 * np.abs(frame).max())
+    _, fp = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, precomp_h=h)
+    assert np.allclose(_grid_of(fp, phy, amp) * h, g, atol=1e-4)
+
+
+def test_tx_frequency_noise_statistics_and_seed():
+    phy, code = _wf_phy_and_code()
+    gen = code.generator()
+    rng = np.random.default_rng(13)
+    info = rng.integers(0, 2, (phy.coded_bits // 128) * 64, dtype=np.uint8)
+    amp, nv = 0.5, 0.04
+    _, clean = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen)
+    _, noisy = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, noise_var=nv, seed=77)
+    _, again = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, noise_var=nv, seed=77)
+    _, other = oracle.tx_frame(info, 128, 64, phy, amplitude=amp, gen_kn=gen, noise_var=nv, seed=78)
+    assert np.array_equal(noisy, again) and not np.array_equal(noisy, other)
+    d = _grid_of(noisy, phy, amp) - _grid_of(clean, phy, amp)
+    assert np.abs(d[phy.pilot_symbols]).max() < 1e-5  # pilots carry no noise
+    z = d[phy.data_symbols].reshape(-1)
+    assert abs(z.mean()) < 4 * np.sqrt(nv / z.size)
+    assert abs(np.var(z.real) - nv / 2) < 0.08 * nv / 2 and abs(np.var(z.imag) - nv / 2) < 0.08 * nv / 2
+    assert abs(np.mean(z.real * z.imag)) < 0.05 * nv / 2
+    from scipy import stats
+
+    assert stats.kstest(z.real / np.sqrt(nv / 2), "norm").pvalue > 1e-3
+    assert stats.kstest(z.imag / np.sqrt(nv / 2), "norm").pvalue > 1e-3
+
+
+def dvbs2_rows(k: int, m: int, per_row: int, seed: int):
+    """Random information part of a DVB-S2-structured parity-check matrix (CSR rows)."""
+    rng = np.random.default_rng(seed)
+    off = [0]
+    cols = []
+    for _ in range(m):
+        c = np.sort(rng.choice(k, per_row, replace=False))
+        cols.extend(c.tolist())
+        off.append(len(cols))
+    return np.asarray(off, np.int32), np.asarray(cols, np.int32)
+
+
+def test_tx_dvbs2_accumulator_satisfies_parity_checks():
+    # H = [A | B], B dual-diagonal: row r checks info cols of row r and parity r, r-1.
+    phy, _ = _wf_phy_and_code()
+    k, n = 64, 128
+    m = n - k
+    off, cols = dvbs2_rows(k, m, 5, 14)
+    H = np.zeros((m, n), np.uint8)
+    for r in range(m):
+        H[r, cols[off[r]:off[r + 1]]] = 1
+        H[r, k + r] = 1
+        if r > 0:
+            H[r, k + r - 1] = 1
+    rng = np.random.default_rng(15)
+    blocks = phy.coded_bits // n
+    info = rng.integers(0, 2, blocks * k, dtype=np.uint8)
+    coded, _ = oracle.tx_frame(info, n, k, phy, amplitude=0.2, dvb_rows=(off, cols))
+    cw = coded[:blocks * n].reshape(blocks, n)
+    assert np.array_equal(cw[:, :k], info.reshape(blocks, k))
+    assert not ((H.astype(np.int64) @ cw.T.astype(np.int64)) % 2).any()
