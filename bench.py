@@ -254,6 +254,11 @@ def leg_mixed(a, sa, np, torch, stream):
 
 
 # --------------------------------------------------------------------- ofdm --
+# Full-preamble refinement peaks in the 6-tap fading channel with CFO fall to
+# 0.2-0.6, so the reference's --sync-threshold is set below its 0.45 default.
+OFDM_SYNC_THRESHOLD = 0.2
+
+
 def leg_ofdm(a, sa, np, torch, stream):
     import waveform as wf
 
@@ -284,7 +289,7 @@ def leg_ofdm(a, sa, np, torch, stream):
     d_info = torch.empty((F * blocks, dec.k), dtype=torch.uint8, device=dev)
     d_ok = torch.empty(F * blocks, dtype=torch.uint8, device=dev)
     sh = stream.cuda_stream
-    rxr = sa.OfdmReceiver(phy, n, blocks, max_search=256, noise_var=nvf)
+    rxr = sa.OfdmReceiver(phy, n, blocks, max_search=256, noise_var=nvf, threshold=OFDM_SYNC_THRESHOLD)
     torch.cuda.synchronize()
 
     def receive():
@@ -306,6 +311,7 @@ def leg_ofdm(a, sa, np, torch, stream):
     fe = int((got != info).reshape(F, -1).any(axis=1).sum())
     alg_bytes = F * (8 * span + 4 * blocks * n)
     out = {"frames": F, "codewords": F * blocks, "code": f"QC n={n} k={dec.k}", "modulation": "16-QAM",
+           "sync_threshold": OFDM_SYNC_THRESHOLD,
            "rx_ms": round(rx_ms, 4), "decode_ms": round(dec_ms, 4), "chain_ms": round(both_ms, 4),
            "frames_per_s": round(F / (both_ms * 1e-3), 1),
            "value": round(F * blocks * dec.k / (both_ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
@@ -315,6 +321,34 @@ def leg_ofdm(a, sa, np, torch, stream):
                            "frac": round(alg_bytes / (rx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                            "bytes_per_launch": alg_bytes}}
     rxr.close()
+    # device-resident test IQ: the batched transmitter (encode + grid + IFFT + compose)
+    tphy = wf.OfdmPhy(bits_per_symbol=4, active_sc=512, num_symbols=14, pilot_symbols=[0])
+    tblocks = tphy.coded_bits // n
+    pl = sa.Pipeline(tphy, n, dec.k, tblocks, slots=1, tx_amplitude=0.2)
+    pi = np.full(tphy.num_symbols, -1, np.int32)
+    di = np.full(tphy.num_symbols, -1, np.int32)
+    pi[tphy.pilot_symbols] = np.arange(len(tphy.pilot_symbols))
+    di[tphy.data_symbols] = np.arange(len(tphy.data_symbols))
+    p0, p1, al = tphy.interp_tables()
+    pl.upload_static_tables(generator_kn=dec.generator(), used_indices=tphy.used_shifted, pilot_symbol_index=pi,
+                            data_symbol_index=di, pilot_symbols=tphy.pilot_symbols, data_symbols=tphy.data_symbols,
+                            interp_p0=p0, interp_p1=p1, interp_alpha=al.astype(np.float32),
+                            pilot_grid=tphy.pilot_grid, preamble=tphy.preamble)
+    t_info = torch.from_numpy(rng.integers(0, 2, (F, tblocks * dec.k), dtype=np.uint8)).to(dev)
+    t_out = torch.empty((F, tphy.frame_len, 2), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def transmit():
+        pl.tx_batch_device(t_info.data_ptr(), F, t_out.data_ptr(), noise_var=0.01, seed=7, stream=sh)
+
+    transmit()
+    torch.cuda.synchronize()
+    tx_ms = timed(transmit, a.ofdm_reps, stream)
+    tx_bytes = F * (8 * tphy.frame_len + tblocks * dec.k + 2 * tphy.coded_bits)
+    out["tx"] = {"frames": F, "modulation": "16-QAM", "active_sc": 512, "code": f"QC n={n} k={dec.k} generator",
+                 "tx_ms": round(tx_ms, 4), "frames_per_s": round(F / (tx_ms * 1e-3), 1),
+                 "achieved_GBs": round(tx_bytes / (tx_ms * 1e-3) / 1e9, 1), "bytes_per_launch": tx_bytes}
+    pl.close()
     dec.close()
     return out
 

@@ -342,6 +342,113 @@ int oracle_nms_decode(int n, int m, const int* row_ptr, const int* col_ind, cons
 }
 
 /* ------------------------------------------------------------------------ */
+/* Layered normalized min-sum (the "colored layered" GPU BP schedule the      */
+/* reference selects for n >= 1000, LINK:3658; its decoder library is absent, */
+/* so the schedule is defined here and pinned by invariants in the tests).    */
+/* Row order: greedy first-fit colouring in increasing row index (row r takes */
+/* the least colour no earlier row sharing a column has), rows sorted by      */
+/* (colour, index).  Rows of one colour share no column, so any order inside  */
+/* a colour gives the same result.  Per row, in CSR edge order:               */
+/*   q_e = APP_v - r_e; min1/min2/first-min/sign exactly as the flooding      */
+/*   check update (LINK:3035-3066); r_e' = alpha * s * mag;                   */
+/*   with damping d: r_e' = (1 - d) * r_e' + d * r_e;  APP_v = q_e + r_e'.    */
+/* After each full pass: hard = APP < 0, syndrome stop (LINK:3070-3080).      */
+/* ------------------------------------------------------------------------ */
+int oracle_layer_order(int n, int m, const int* row_ptr, const int* col_ind, int* order_out,
+                       int* color_out)
+{
+    int* cdeg = (int*)calloc((size_t)n + 1, sizeof(int));
+    const int E = row_ptr[m];
+    for (int e = 0; e < E; ++e) cdeg[col_ind[e] + 1]++;
+    for (int c = 0; c < n; ++c) cdeg[c + 1] += cdeg[c];
+    int* crow = (int*)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    int* fill = (int*)calloc((size_t)n, sizeof(int));
+    for (int r = 0; r < m; ++r)
+        for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) crow[cdeg[col_ind[e]] + fill[col_ind[e]]++] = r;
+    int* color = (int*)malloc(sizeof(int) * (size_t)m);
+    int* mark = (int*)calloc((size_t)m + 1, sizeof(int)); /* mark[c] == r + 1: colour c used by a neighbour of r */
+    int ncolors = 0;
+    for (int r = 0; r < m; ++r) {
+        for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+            const int c = col_ind[e];
+            for (int j = cdeg[c]; j < cdeg[c + 1]; ++j) {
+                const int r2 = crow[j];
+                if (r2 < r) mark[color[r2]] = r + 1;
+            }
+        }
+        int col = 0;
+        while (mark[col] == r + 1) ++col;
+        color[r] = col;
+        if (col + 1 > ncolors) ncolors = col + 1;
+    }
+    int pos = 0;
+    for (int c = 0; c < ncolors; ++c)
+        for (int r = 0; r < m; ++r)
+            if (color[r] == c) order_out[pos++] = r;
+    if (color_out) memcpy(color_out, color, sizeof(int) * (size_t)m);
+    free(cdeg);
+    free(crow);
+    free(fill);
+    free(color);
+    free(mark);
+    return ncolors;
+}
+
+int oracle_nms_layered_decode(int n, int m, const int* row_ptr, const int* col_ind, const int* order,
+                              const float* llr, int max_iter, float alpha, float offset, float damping,
+                              int early_stop, uint8_t* hard_out, float* app_out, int* iters_out)
+{
+    const int E = row_ptr[m];
+    float* r = (float*)calloc((size_t)(E > 0 ? E : 1), sizeof(float));
+    float* q = (float*)malloc(sizeof(float) * (size_t)(E > 0 ? E : 1));
+    float* app = (float*)malloc(sizeof(float) * (size_t)n);
+    uint8_t* hard = (uint8_t*)malloc((size_t)n);
+    memcpy(app, llr, sizeof(float) * (size_t)n);
+    int ok = 0, used = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        used = it + 1;
+        for (int i = 0; i < m; ++i) {
+            const int row = order[i];
+            float min1 = FLT_MAX, min2 = FLT_MAX;
+            int min1_edge = -1, sign_prod = 1;
+            for (int e = row_ptr[row]; e < row_ptr[row + 1]; ++e) {
+                const float v = app[col_ind[e]] - r[e];
+                q[e] = v;
+                if (v < 0.0f) sign_prod = -sign_prod;
+                const float av = fabsf(v);
+                if (av < min1) {
+                    min2 = min1;
+                    min1 = av;
+                    min1_edge = e;
+                } else if (av < min2) {
+                    min2 = av;
+                }
+            }
+            for (int e = row_ptr[row]; e < row_ptr[row + 1]; ++e) {
+                const int s = (q[e] < 0.0f) ? -sign_prod : sign_prod;
+                float mag = (e == min1_edge) ? min2 : min1;
+                if (offset != 0.0f) mag = fmaxf(mag - offset, 0.0f);
+                float rn = alpha * (float)s * mag;
+                if (damping != 0.0f) rn = (1.0f - damping) * rn + damping * r[e];
+                r[e] = rn;
+                app[col_ind[e]] = q[e] + rn;
+            }
+        }
+        for (int c = 0; c < n; ++c) hard[c] = app[c] < 0.0f ? 1 : 0;
+        ok = syndrome_weight(m, row_ptr, col_ind, hard) == 0;
+        if (ok && early_stop) break;
+    }
+    if (hard_out) memcpy(hard_out, hard, (size_t)n);
+    if (app_out) memcpy(app_out, app, sizeof(float) * (size_t)n);
+    if (iters_out) *iters_out = used;
+    free(r);
+    free(q);
+    free(app);
+    free(hard);
+    return ok;
+}
+
+/* ------------------------------------------------------------------------ */
 /* OSD: MOSD:187-245 (osd_decode), MOSD:247-250 (encode_and_metric),         */
 /* MOSD:270-305 (generator_to_systematic).  Dense byte matrices, literal     */
 /* restatement.  MATLAB sort is stable (ties keep the lower index first).    */

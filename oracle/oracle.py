@@ -101,6 +101,12 @@ class Code:
         return {"ok": bool(ok), "hard": hard, "app": app, "best_app": best, "iterations": it.value,
                 "best_synw": bw.value}
 
+    def layer_order(self):
+        return layer_order(self.n, self.m, self.rp, self.ci)
+
+    def nms_layered(self, llr, order=None, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, early_stop=True):
+        return nms_layered(self.n, self.m, self.rp, self.ci, llr, order, max_iter, alpha, offset, damping, early_stop)
+
     def bposd(self, llr, gen=None, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, order=1, max_flip=64,
               max_tests=4096, threads=1):
         llr = np.ascontiguousarray(llr, np.float32).reshape(-1, self.n)
@@ -142,6 +148,33 @@ def awgn_llr(codewords: np.ndarray, ebn0_db: float, rate: float, rng: np.random.
     x = 1.0 - 2.0 * codewords.astype(np.float64)
     y = x + np.sqrt(sigma2) * rng.standard_normal(codewords.shape)
     return (2.0 / sigma2 * y).astype(np.float32)
+
+
+def layer_order(n, m, rp, ci):
+    """Greedy first-fit row colouring -> (rows sorted by (colour, row), colour of each row)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    order = np.zeros(m, np.int32)
+    color = np.zeros(m, np.int32)
+    load().oracle_layer_order(n, m, _p(rp, i32p), _p(ci, i32p), _p(order, i32p), _p(color, i32p))
+    return order, color
+
+
+def nms_layered(n, m, rp, ci, llr, order=None, max_iter=50, alpha=0.75, offset=0.0, damping=0.0, early_stop=True):
+    """Layered normalized min-sum over rows in `order` (default: the greedy colouring order)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    if order is None:
+        order = layer_order(n, m, rp, ci)[0]
+    order = np.ascontiguousarray(order, np.int32)
+    llr = np.ascontiguousarray(llr, np.float32)
+    hard = np.zeros(n, np.uint8)
+    app = np.zeros(n, np.float32)
+    it = C.c_int()
+    ok = load().oracle_nms_layered_decode(n, m, _p(rp, i32p), _p(ci, i32p), _p(order, i32p), _p(llr, f32p),
+                                          max_iter, C.c_float(alpha), C.c_float(offset), C.c_float(damping),
+                                          int(early_stop), _p(hard, u8p), _p(app, f32p), C.byref(it))
+    return {"ok": bool(ok), "hard": hard, "app": app, "iterations": it.value}
 
 
 # ---------------------------------------------------------------- OFDM ----

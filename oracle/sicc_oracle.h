@@ -57,6 +57,15 @@ int oracle_osd_decode(int n, int k, const uint8_t* gen, const float* llr, int or
                       int max_flip, int max_tests, uint8_t* code_out, double* metric_out,
                       int* tests_out);
 
+/* ---- layered normalized min-sum (GPU "colored layered" schedule) ----
+ * oracle_layer_order: greedy first-fit row colouring in row order; order_out
+ * [m] = rows sorted by (colour, row), color_out [m] optional; returns the
+ * number of colours.  oracle_nms_layered_decode processes rows in `order`. */
+int oracle_layer_order(int n, int m, const int* row_ptr, const int* col_ind, int* order_out, int* color_out);
+int oracle_nms_layered_decode(int n, int m, const int* row_ptr, const int* col_ind, const int* order,
+                              const float* llr, int max_iter, float alpha, float offset, float damping,
+                              int early_stop, uint8_t* hard_out, float* app_out, int* iters_out);
+
 /* ---- BP followed by OSD on failure, over a batch (MOSD:55-77) ---- */
 int oracle_bposd_batch(int count, int n, int m, int k, const int* row_ptr, const int* col_ind,
                        const uint8_t* gen, const int* info_cols, const float* llr, int max_iter,
