@@ -216,7 +216,7 @@ def leg_mixed(a, sa, np, torch, stream):
     info_bits_total = 0
     for i, (n, mb) in enumerate(shapes):
         m, rp, ci = sa.qc_code(n, mb, 3, seed=1000 + n + mb)
-        dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=50, alpha=0.75)
+        dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=50, alpha=0.75, schedule=0)
         cnt = int((draw == i).sum())
         info, llr = gen_llr(dec.generator(), cnt, MIX_EBN0[(n, mb)], dec.k / n, rng)
         d_llr = torch.from_numpy(llr).to(dev)
@@ -269,7 +269,7 @@ def leg_ofdm(a, sa, np, torch, stream):
     n = 1024
     blocks = phy.coded_bits // n
     m, rp, ci = sa.qc_code(n, 8, 3, seed=64)
-    dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75)
+    dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0)
     info = rng.integers(0, 2, (F * blocks, dec.k), dtype=np.uint8)
     cw = ((info.astype(np.float32) @ dec.generator().astype(np.float32)) % 2).astype(np.uint8)
     coded = np.concatenate([cw.reshape(F, blocks * n),

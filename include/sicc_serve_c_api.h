@@ -47,8 +47,13 @@ int sicc_server_poll(unsigned long long handle, int max_jobs, unsigned long long
 int sicc_server_fetch(unsigned long long handle, unsigned long long job_id, unsigned char* info_bits,
                       unsigned char* bp_success, unsigned char* used_osd, double* latency_us);
 
-/* Block until every submitted job has completed. */
+/* Block until every submitted job has completed (an error while held with
+ * jobs pending). */
 int sicc_server_drain(unsigned long long handle);
+
+/* hold != 0: queue arrivals without admitting or launching them (launches in
+ * flight complete); hold == 0: resume, so the queued burst is packed. */
+int sicc_server_hold(unsigned long long handle, int hold);
 
 /* Native load driver: submits job j at arrival_us[j] (microseconds after the
  * call starts, busy-waiting on a steady clock) from device LLR sources, waits

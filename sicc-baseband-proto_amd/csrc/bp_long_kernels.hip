@@ -1,0 +1,226 @@
+// bp_long_kernels.hip -- normalized min-sum BP for long codes (DVB-S2 N16200 /
+// N64800 and any CSR code the short-code kernel cannot hold), flooding or
+// layered schedule.
+//
+// One workgroup of 16 wavefronts (8 for row degrees above 16) per codeword, persistent over
+// the batch.  APP values live in LDS when n * 4 bytes fit in 64 KiB (n <= 16 368),
+// otherwise in a per-workgroup slice of global scratch (L2 / MALL resident).
+// Check-to-variable messages r live in global scratch in a group-major,
+// position-minor layout: group g's rows are processed by consecutive threads,
+// and edge j of the row at position p is slot grp_edge[g] + j * rows_g + p, so a
+// wavefront's loads of one edge index are contiguous.
+//
+// Layered (schedule 2 / 3): the groups are the colours of a greedy first-fit
+// row colouring; rows of a colour share no column and run in parallel, colours
+// run in order.  Each row: q = APP - r_old, the min-sum check update of the
+// flooding decoder (LINK:3035-3066), r' = alpha * s * mag, optional damping
+// r' = (1 - d) r' + d r_old, APP = q + r'.
+// Flooding (schedule 0 / 1): one group of all rows in row order; check phase
+// from q = APP - r_old (damped against the stored q when damping != 0, as
+// MOSD:174-177), then each column sums llr + r over its edges in increasing row
+// order (LINK:3068-3074) -- bit-identical to the short-code kernel's result.
+// Both: syndrome stop after every iteration; a frame is accepted when the
+// syndrome is zero and min |APP| >= min_abs_accept.
+#include "kernels.hpp"
+
+#include <cfloat>
+#include <stdexcept>
+#include <string>
+
+namespace sicc {
+
+namespace {
+
+// 16 wavefronts per codeword; rows of degree above 16 keep 32 messages in
+// registers, which needs the larger register budget of 8 wavefronts.
+template <int MAXW>
+constexpr int long_threads()
+{
+    return MAXW > 16 ? 512 : 1024;
+}
+
+template <int MAXW, bool LAYERED, bool APP_LDS>
+__global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCodeDev c, BpParams p, BpIo io,
+                                                                       BpLongScratch sc)
+{
+    constexpr int kLongThreads = long_threads<MAXW>();
+    extern __shared__ float lds_app[];
+    __shared__ int s_flag[2];
+    __shared__ float s_min[16];
+    const int tid = threadIdx.x;
+    float* r = sc.r + static_cast<size_t>(blockIdx.x) * c.slots;
+    float* qst = sc.q ? sc.q + static_cast<size_t>(blockIdx.x) * c.slots : nullptr;
+    float* app = APP_LDS ? lds_app : sc.app + static_cast<size_t>(blockIdx.x) * c.n;
+    const float alpha = p.alpha, offset = p.offset, damping = p.damping;
+
+    for (int cw = blockIdx.x; cw < io.num_cw; cw += gridDim.x) {
+        const float* llr = io.llr + static_cast<size_t>(cw) * c.n;
+        for (int v = tid; v < c.n; v += kLongThreads) app[v] = llr[v];
+        for (int e = tid; e < c.slots; e += kLongThreads) r[e] = 0.0f;
+        if (tid < 2) s_flag[tid] = 0;
+        __syncthreads();
+        int it = 0;
+        bool ok = false;
+        while (it < p.max_iter) {
+            for (int g = 0; g < c.groups; ++g) {
+                const int rows = c.grp_row[g + 1] - c.grp_row[g];
+                const int base = c.grp_edge[g];
+                const int width = c.grp_width[g];
+                for (int pos = tid; pos < rows; pos += kLongThreads) {
+                    float qv[MAXW], rv[MAXW];
+                    int cv[MAXW];
+                    float m1 = FLT_MAX, m2 = FLT_MAX;
+                    int mi = -1, neg = 0, deg = 0;
+#pragma unroll
+                    for (int j = 0; j < MAXW; ++j) {
+                        if (j < width) {
+                            const int slot = base + j * rows + pos;
+                            const int col = c.edge_col[slot];
+                            if (col >= 0) {
+                                const float rold = r[slot];
+                                float q = app[col] - rold;
+                                if (!LAYERED && damping != 0.0f) {
+                                    if (it > 0) q = (1.0f - damping) * q + damping * qst[slot];
+                                    qst[slot] = q;
+                                }
+                                qv[j] = q;
+                                rv[j] = rold;
+                                cv[j] = col;
+                                deg = j + 1;
+                                neg ^= q < 0.0f ? 1 : 0;
+                                const float av = fabsf(q);
+                                if (av < m1) {
+                                    m2 = m1;
+                                    m1 = av;
+                                    mi = j;
+                                } else if (av < m2) {
+                                    m2 = av;
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < MAXW; ++j) {
+                        if (j < deg) {
+                            const int s = ((qv[j] < 0.0f ? 1 : 0) ^ neg) ? -1 : 1;
+                            float mag = (j == mi) ? m2 : m1;
+                            if (offset != 0.0f) mag = fmaxf(mag - offset, 0.0f);
+                            float rn = alpha * static_cast<float>(s) * mag;
+                            if (LAYERED) {
+                                if (damping != 0.0f) rn = (1.0f - damping) * rn + damping * rv[j];
+                                app[cv[j]] = qv[j] + rn;
+                            }
+                            r[base + j * rows + pos] = rn;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (!LAYERED) {
+                for (int v = tid; v < c.n; v += kLongThreads) {
+                    float s = llr[v];
+                    for (int t = c.col_ptr[v]; t < c.col_ptr[v + 1]; ++t) s += r[c.col_slot[t]];
+                    app[v] = s;
+                }
+                __syncthreads();
+            }
+            // syndrome of the hard decision APP < 0
+            const int fl = it & 1;
+            for (int g = 0; g < c.groups; ++g) {
+                const int rows = c.grp_row[g + 1] - c.grp_row[g];
+                const int base = c.grp_edge[g];
+                const int width = c.grp_width[g];
+                for (int pos = tid; pos < rows; pos += kLongThreads) {
+                    int par = 0;
+                    for (int j = 0; j < width; ++j) {
+                        const int col = c.edge_col[base + j * rows + pos];
+                        if (col < 0) break;
+                        par ^= app[col] < 0.0f ? 1 : 0;
+                    }
+                    if (par) s_flag[fl] = 1;
+                }
+            }
+            if (tid == 0) s_flag[fl ^ 1] = 0;
+            __syncthreads();
+            ok = s_flag[fl] == 0;
+            ++it;
+            if (ok) break;
+        }
+        bool accept = ok;
+        if (ok && p.min_abs_accept > 0.0f) {
+            float mn = FLT_MAX;
+            for (int v = tid; v < c.n; v += kLongThreads) mn = fminf(mn, fabsf(app[v]));
+            for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
+            if ((tid & 63) == 0) s_min[tid >> 6] = mn;
+            __syncthreads();
+            mn = FLT_MAX;
+            for (int w = 0; w < kLongThreads / 64; ++w) mn = fminf(mn, s_min[w]);
+            accept = mn >= p.min_abs_accept;
+        }
+        const size_t ob = static_cast<size_t>(cw) * c.n;
+        for (int v = tid; v < c.n; v += kLongThreads) {
+            const float a = app[v];
+            if (io.code_bits) io.code_bits[ob + v] = a < 0.0f ? 1 : 0;
+            if (io.app) io.app[ob + v] = a;
+        }
+        if (io.info_bits)
+            for (int i = tid; i < c.k; i += kLongThreads)
+                io.info_bits[static_cast<size_t>(cw) * c.k + i] = app[c.info_cols[i]] < 0.0f ? 1 : 0;
+        if (tid == 0) {
+            if (io.success) io.success[cw] = accept ? 1 : 0;
+            if (io.iterations) io.iterations[cw] = it;
+            if (io.used_osd) io.used_osd[cw] = 0;
+            if (io.distance) io.distance[cw] = 0.0f;
+        }
+        __syncthreads();  // APP and flags are reused by the next codeword
+    }
+}
+
+template <int MAXW>
+void launch_width(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, const BpLongScratch& sc, int grid,
+                  hipStream_t s)
+{
+    const size_t lds = sc.app_in_lds ? sizeof(float) * static_cast<size_t>(c.n) : 0;
+    constexpr int kLongThreads = long_threads<MAXW>();
+    if (c.layered) {
+        if (sc.app_in_lds)
+            hipLaunchKernelGGL((bp_long_kernel<MAXW, true, true>), dim3(grid), dim3(kLongThreads), lds, s, c, p, io, sc);
+        else
+            hipLaunchKernelGGL((bp_long_kernel<MAXW, true, false>), dim3(grid), dim3(kLongThreads), 0, s, c, p, io, sc);
+    } else {
+        if (sc.app_in_lds)
+            hipLaunchKernelGGL((bp_long_kernel<MAXW, false, true>), dim3(grid), dim3(kLongThreads), lds, s, c, p, io,
+                               sc);
+        else
+            hipLaunchKernelGGL((bp_long_kernel<MAXW, false, false>), dim3(grid), dim3(kLongThreads), 0, s, c, p, io,
+                               sc);
+    }
+}
+
+}  // namespace
+
+bool bp_long_app_fits_lds(int n) { return n > 0 && static_cast<size_t>(n) * sizeof(float) <= 65536 - 256; }
+
+void launch_bp_long(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, const BpLongScratch& sc,
+                    hipStream_t s)
+{
+    if (io.num_cw <= 0) return;
+    if (!io.llr || !sc.r || sc.blocks <= 0) throw std::invalid_argument("long BP: missing input or scratch");
+    if (!sc.app_in_lds && !sc.app) throw std::invalid_argument("long BP: missing APP scratch");
+    if (sc.app_in_lds && !bp_long_app_fits_lds(c.n)) throw std::invalid_argument("long BP: APP does not fit LDS");
+    if (!c.layered && p.damping != 0.0f && !sc.q) throw std::invalid_argument("long BP: damping needs q scratch");
+    if (p.max_iter < 1) throw std::invalid_argument("long BP: max_iter must be positive");
+    const int grid = io.num_cw < sc.blocks ? io.num_cw : sc.blocks;
+    if (c.max_width <= 8)
+        launch_width<8>(c, p, io, sc, grid, s);
+    else if (c.max_width <= 16)
+        launch_width<16>(c, p, io, sc, grid, s);
+    else if (c.max_width <= 32)
+        launch_width<32>(c, p, io, sc, grid, s);
+    else
+        throw std::invalid_argument("long BP: row degree above 32");
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("bp_long_kernel: ") + hipGetErrorString(e));
+}
+
+}  // namespace sicc

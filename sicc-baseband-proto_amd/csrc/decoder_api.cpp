@@ -443,6 +443,7 @@ int cuda_osd_create_bp_csr_decoder(const cuda_osd_bp_csr_config_t* config, const
         d->bp.damping = config->damping;
         d->bp.min_abs_accept = config->min_abs_llr_accept;
         d->schedule = config->schedule;
+        d->bp.schedule = config->schedule;
         d->osd_order = -1;
         d->max_batch = std::max(config->max_batch_size, 1);
         *out_handle = table().add(std::move(d));
@@ -881,7 +882,8 @@ int sicc_multi_decode_device(unsigned long long handle, const sicc_group_t* grou
             for (int i = 0; i < n_groups; ++i) {
                 Decoder& d = *md->members[groups[i].member];
                 d.eng->set_timing(false);
-                if (d.kind == Kind::Stream && d.osd_order >= 0) d.eng->reserve(groups[i].count);
+                if ((d.kind == Kind::Stream && d.osd_order >= 0) || d.eng->uses_long_kernel(d.bp))
+                    d.eng->reserve(groups[i].count, &d.bp);
             }
             SICC_HIP(hipStreamSynchronize(md->main));
             SICC_HIP(hipStreamBeginCapture(md->main, hipStreamCaptureModeThreadLocal));

@@ -12,51 +12,61 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
     const int n = code_.n, m = code_.m, E = code_.edges();
     int max_dc = 0;
     for (int r = 0; r < m; ++r) max_dc = std::max(max_dc, code_.row_ptr[r + 1] - code_.row_ptr[r]);
-    if (n > 0x7fff || E > 0xffff) throw std::invalid_argument("code too large for the short-code BP tables");
-
-    // Column-major slots: a column's edges in increasing row order.
-    std::vector<int> col_cnt(n + 1, 0);
-    for (int e = 0; e < E; ++e) col_cnt[code_.col_ind[e] + 1]++;
-    for (int v = 0; v < n; ++v) col_cnt[v + 1] += col_cnt[v];
-    std::vector<int> fill(n, 0), slot(E);
-    for (int r = 0; r < m; ++r)
-        for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e) {
-            const int v = code_.col_ind[e];
-            slot[e] = col_cnt[v] + fill[v]++;
-        }
-    int max_dv = 0;
-    std::vector<int> col_meta(n);
-    for (int v = 0; v < n; ++v) {
-        const int dv = col_cnt[v + 1] - col_cnt[v];
-        max_dv = std::max(max_dv, dv);
-        if (dv > 255) throw std::invalid_argument("variable degree above 255");
-        col_meta[v] = (col_cnt[v] << 8) | dv;
-    }
-    std::vector<int> row_meta(static_cast<size_t>(m) * std::max(max_dc, 1), -1);
-    for (int r = 0; r < m; ++r) {
-        int j = 0;
-        for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e, ++j)
-            row_meta[static_cast<size_t>(r) * max_dc + j] = (code_.col_ind[e] << 16) | slot[e];
-    }
-    row_meta_.alloc(row_meta.size());
-    col_meta_.alloc(n);
     info_cols_.alloc(std::max(code_.k, 1));
-    SICC_HIP(hipMemcpy(row_meta_.get(), row_meta.data(), row_meta.size() * sizeof(int), hipMemcpyHostToDevice));
-    SICC_HIP(hipMemcpy(col_meta_.get(), col_meta.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    SICC_HIP(hipMemcpy(info_cols_.get(), code_.info_cols.data(), code_.k * sizeof(int), hipMemcpyHostToDevice));
+    if (code_.k > 0)
+        SICC_HIP(hipMemcpy(info_cols_.get(), code_.info_cols.data(), code_.k * sizeof(int), hipMemcpyHostToDevice));
+    int dev = 0;
+    SICC_HIP(hipGetDevice(&dev));
+    SICC_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+    cus_ = std::max(cus_, 1);
+    max_waves_ = cus_ * 32;
+    own_.ensure(1, n);
 
-    bpdev_.n = n;
-    bpdev_.m = m;
-    bpdev_.k = code_.k;
-    bpdev_.E = E;
-    bpdev_.max_dc = max_dc;
-    bpdev_.row_meta = row_meta_.get();
-    bpdev_.col_meta = col_meta_.get();
-    bpdev_.info_cols = info_cols_.get();
-    if (!bp_shape_supported(bpdev_)) throw std::invalid_argument("code shape not supported by the BP kernel");
-    tune_.team = bp_default_team(bpdev_);
+    // Short-code tables: (col << 16) | slot row entries and (slot << 8) | degree
+    // column entries bound n, E and the variable degree.
+    short_ok_ = n <= 0x7fff && E <= 0xffff;
+    if (short_ok_) {
+        std::vector<int> col_cnt(n + 1, 0);
+        for (int e = 0; e < E; ++e) col_cnt[code_.col_ind[e] + 1]++;
+        for (int v = 0; v < n; ++v) col_cnt[v + 1] += col_cnt[v];
+        for (int v = 0; v < n && short_ok_; ++v) short_ok_ = col_cnt[v + 1] - col_cnt[v] <= 255;
+        if (short_ok_) {
+            // Column-major slots: a column's edges in increasing row order.
+            std::vector<int> fill(n, 0), slot(E);
+            for (int r = 0; r < m; ++r)
+                for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e) {
+                    const int v = code_.col_ind[e];
+                    slot[e] = col_cnt[v] + fill[v]++;
+                }
+            std::vector<int> col_meta(n);
+            for (int v = 0; v < n; ++v) col_meta[v] = (col_cnt[v] << 8) | (col_cnt[v + 1] - col_cnt[v]);
+            std::vector<int> row_meta(static_cast<size_t>(m) * std::max(max_dc, 1), -1);
+            for (int r = 0; r < m; ++r) {
+                int j = 0;
+                for (int e = code_.row_ptr[r]; e < code_.row_ptr[r + 1]; ++e, ++j)
+                    row_meta[static_cast<size_t>(r) * max_dc + j] = (code_.col_ind[e] << 16) | slot[e];
+            }
+            bpdev_.n = n;
+            bpdev_.m = m;
+            bpdev_.k = code_.k;
+            bpdev_.E = E;
+            bpdev_.max_dc = max_dc;
+            short_ok_ = bp_shape_supported(bpdev_);
+            if (short_ok_) {
+                row_meta_.alloc(row_meta.size());
+                col_meta_.alloc(n);
+                SICC_HIP(hipMemcpy(row_meta_.get(), row_meta.data(), row_meta.size() * sizeof(int),
+                                   hipMemcpyHostToDevice));
+                SICC_HIP(hipMemcpy(col_meta_.get(), col_meta.data(), n * sizeof(int), hipMemcpyHostToDevice));
+                bpdev_.row_meta = row_meta_.get();
+                bpdev_.col_meta = col_meta_.get();
+                bpdev_.info_cols = info_cols_.get();
+                tune_.team = bp_default_team(bpdev_);
+            }
+        }
+    }
 
-    osd_ok_ = osd_shape_supported(n, code_.k);
+    osd_ok_ = short_ok_ && osd_shape_supported(n, code_.k);
     if (osd_ok_) {
         const auto g = code_.dense_generator();
         std::vector<unsigned long long> cols(n, 0);
@@ -70,12 +80,172 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
         osddev_.gen_cols = gen_cols_.get();
         osddev_.info_cols = info_cols_.get();
     }
-    own_.ensure(1, n);
+}
 
-    int dev = 0, cus = 256;
-    SICC_HIP(hipGetDevice(&dev));
-    SICC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    max_waves_ = std::max(cus, 1) * 32;
+bool Engine::uses_long_kernel(const BpParams& bp) const
+{
+    const bool layered = bp.schedule >= 2 || (bp.schedule < 0 && code_.n >= 1000);
+    return layered || !short_ok_;
+}
+
+// Greedy first-fit colouring of the rows in row order (row r takes the least
+// colour no earlier row sharing a column has); layered groups are the colours,
+// rows sorted by (colour, row).  Flooding: one group of all rows in row order.
+const LongTables& Engine::long_tables(bool layered)
+{
+    auto& slot = long_[layered ? 1 : 0];
+    if (slot) return *slot;
+    const int n = code_.n, m = code_.m, E = code_.edges();
+    const auto& rp = code_.row_ptr;
+    const auto& ci = code_.col_ind;
+    std::vector<int> order(m), grp_row{0};
+    if (layered) {
+        std::vector<int> cptr(n + 1, 0), crow(E), fill(n, 0);
+        for (int e = 0; e < E; ++e) cptr[ci[e] + 1]++;
+        for (int v = 0; v < n; ++v) cptr[v + 1] += cptr[v];
+        for (int r = 0; r < m; ++r)
+            for (int e = rp[r]; e < rp[r + 1]; ++e) crow[cptr[ci[e]] + fill[ci[e]]++] = r;
+        std::vector<int> color(m, 0), mark(m + 1, 0);
+        int ncol = 0;
+        for (int r = 0; r < m; ++r) {
+            for (int e = rp[r]; e < rp[r + 1]; ++e) {
+                const int v = ci[e];
+                for (int t = cptr[v]; t < cptr[v + 1]; ++t) {
+                    const int r2 = crow[t];
+                    if (r2 < r) mark[color[r2]] = r + 1;
+                    if (r2 == r && t > cptr[v] && crow[t - 1] == r)
+                        throw std::invalid_argument("layered BP needs rows without repeated columns");
+                }
+            }
+            int c = 0;
+            while (mark[c] == r + 1) ++c;
+            color[r] = c;
+            ncol = std::max(ncol, c + 1);
+        }
+        std::vector<int> cnt(ncol + 1, 0);
+        for (int r = 0; r < m; ++r) cnt[color[r] + 1]++;
+        for (int c = 0; c < ncol; ++c) cnt[c + 1] += cnt[c];
+        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+        for (int r = 0; r < m; ++r) order[pos[color[r]]++] = r;
+        grp_row.assign(cnt.begin(), cnt.end());
+    } else {
+        for (int r = 0; r < m; ++r) order[r] = r;
+        grp_row.push_back(m);
+    }
+    const int groups = static_cast<int>(grp_row.size()) - 1;
+    std::vector<int> grp_edge(groups), grp_width(groups);
+    long long slots = 0;
+    int max_width = 0;
+    for (int g = 0; g < groups; ++g) {
+        int w = 0;
+        for (int i = grp_row[g]; i < grp_row[g + 1]; ++i) w = std::max(w, rp[order[i] + 1] - rp[order[i]]);
+        grp_edge[g] = static_cast<int>(slots);
+        grp_width[g] = w;
+        max_width = std::max(max_width, w);
+        slots += static_cast<long long>(w) * (grp_row[g + 1] - grp_row[g]);
+        if (slots > (1ll << 30)) throw std::invalid_argument("code too large for the long-code BP tables");
+    }
+    if (max_width > 32) throw std::invalid_argument("long-code BP supports row degrees up to 32");
+    std::vector<int> edge_col(static_cast<size_t>(std::max<long long>(slots, 1)), -1);
+    std::vector<int> slot_of_edge(E);
+    for (int g = 0; g < groups; ++g) {
+        const int rows = grp_row[g + 1] - grp_row[g];
+        for (int p = 0; p < rows; ++p) {
+            const int r = order[grp_row[g] + p];
+            for (int e = rp[r]; e < rp[r + 1]; ++e) {
+                const int sl = grp_edge[g] + (e - rp[r]) * rows + p;
+                edge_col[sl] = ci[e];
+                slot_of_edge[e] = sl;
+            }
+        }
+    }
+    auto t = std::make_unique<LongTables>();
+    auto up = [](DevBuf<int>& b, const std::vector<int>& v) {
+        b.alloc(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) SICC_HIP(hipMemcpy(b.get(), v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    };
+    up(t->grp_row, grp_row);
+    up(t->grp_edge, grp_edge);
+    up(t->grp_width, grp_width);
+    up(t->edge_col, edge_col);
+    if (!layered) {
+        // column v's slots in increasing row order (edges of a row appear once per row)
+        std::vector<int> cptr(n + 1, 0), cslot(E), fill(n, 0);
+        for (int e = 0; e < E; ++e) cptr[ci[e] + 1]++;
+        for (int v = 0; v < n; ++v) cptr[v + 1] += cptr[v];
+        for (int r = 0; r < m; ++r)
+            for (int e = rp[r]; e < rp[r + 1]; ++e) cslot[cptr[ci[e]] + fill[ci[e]]++] = slot_of_edge[e];
+        up(t->col_ptr, cptr);
+        up(t->col_slot, cslot);
+    }
+    auto& d = t->dev;
+    d.n = n;
+    d.m = m;
+    d.k = code_.k;
+    d.slots = static_cast<int>(slots);
+    d.groups = groups;
+    d.max_width = max_width;
+    d.layered = layered ? 1 : 0;
+    d.grp_row = t->grp_row.get();
+    d.grp_edge = t->grp_edge.get();
+    d.grp_width = t->grp_width.get();
+    d.edge_col = t->edge_col.get();
+    d.col_ptr = layered ? nullptr : t->col_ptr.get();
+    d.col_slot = layered ? nullptr : t->col_slot.get();
+    d.info_cols = info_cols_.get();
+    slot = std::move(t);
+    return *slot;
+}
+
+BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool layered, DecodeScratch& sc)
+{
+    BpLongScratch ls;
+    ls.blocks = cus_ * 2;
+    ls.app_in_lds = bp_long_app_fits_lds(code_.n);
+    const bool need_q = !layered && bp.damping != 0.0f;
+    sc.ensure_long(ls.blocks, t.dev.slots, code_.n, need_q, !ls.app_in_lds);
+    ls.r = sc.long_r.get();
+    ls.q = need_q ? sc.long_q.get() : nullptr;
+    ls.app = ls.app_in_lds ? nullptr : sc.long_app.get();
+    return ls;
+}
+
+void Engine::reserve(int num_cw, const BpParams* bp)
+{
+    own_.ensure(num_cw, code_.n);
+    if (bp && uses_long_kernel(*bp)) {
+        const bool layered = bp->schedule >= 2 || (bp->schedule < 0 && code_.n >= 1000);
+        long_scratch(long_tables(layered), *bp, layered, own_);
+    }
+}
+
+void Engine::decode_long(const float* d_llr, int num_cw, const BpParams& bp, bool layered, const DecodeOut& out,
+                         hipStream_t s, DecodeScratch& sc)
+{
+    const LongTables& t = long_tables(layered);
+    const BpLongScratch ls = long_scratch(t, bp, layered, sc);
+    BpIo io;
+    io.llr = d_llr;
+    io.num_cw = num_cw;
+    io.code_bits = out.code;
+    io.info_bits = out.info;
+    io.app = out.app;
+    io.success = out.bp_ok;
+    io.iterations = out.iterations;
+    io.used_osd = out.used_osd;
+    io.distance = out.distance;
+    timed_begin(0, s);
+    launch_bp_long(t.dev, bp, io, ls, s);
+    timed_end(s);
+}
+
+void DecodeScratch::ensure_long(int blocks, int slots, int n, bool need_q, bool need_app)
+{
+    const size_t rs = static_cast<size_t>(blocks) * slots;
+    if (long_r.size() < rs) long_r.alloc(rs);
+    if (need_q && long_q.size() < rs) long_q.alloc(rs);
+    const size_t as = static_cast<size_t>(blocks) * n;
+    if (need_app && long_app.size() < as) long_app.alloc(as);
 }
 
 void DecodeScratch::ensure(int num_cw, int n)
@@ -96,6 +266,11 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
     if (num_cw <= 0) return;
     if (!d_llr) throw std::invalid_argument("null LLR pointer");
     const bool use_osd = osd_order >= 0;
+    if (uses_long_kernel(bp)) {
+        if (use_osd) throw std::invalid_argument("BP-OSD runs the flooding schedule of codes with n <= 256");
+        decode_long(d_llr, num_cw, bp, bp.schedule >= 2 || (bp.schedule < 0 && code_.n >= 1000), out, s, sc);
+        return;
+    }
     if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 64, n <= 256)");
     BpIo io;
     io.llr = d_llr;
@@ -158,6 +333,7 @@ void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, cons
 
 void Engine::set_tune(const BpTune& t)
 {
+    if (!short_ok_) throw std::invalid_argument("launch shapes apply to the short-code BP kernel");
     const int team = t.team > 0 ? t.team : bp_default_team(bpdev_);
     if (!bp_team_supported(bpdev_, team)) throw std::invalid_argument("BP launch shape not supported for this code");
     if (t.teams_per_block != 1 && t.teams_per_block != 2 && t.teams_per_block != 4)
@@ -170,6 +346,7 @@ void Engine::set_tune(const BpTune& t)
 double Engine::autotune(const float* d_llr, int num_cw, const BpParams& bp, hipStream_t s)
 {
     if (!d_llr || num_cw <= 0) throw std::invalid_argument("autotune needs a sample batch");
+    if (uses_long_kernel(bp)) throw std::invalid_argument("autotune applies to the flooding short-code BP kernel");
     DevBuf<uint8_t> bits(static_cast<size_t>(num_cw) * code_.k);
     DevBuf<uint8_t> ok(num_cw);
     BpIo io;

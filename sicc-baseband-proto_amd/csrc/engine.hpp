@@ -8,6 +8,7 @@
 #include "kernels.hpp"
 #include "sicc_common.hpp"
 
+#include <memory>
 #include <vector>
 
 namespace sicc {
@@ -31,6 +32,16 @@ struct DecodeScratch {
     DevBuf<int> counters;
     int cap = 0;
     void ensure(int num_cw, int n);
+    // long-code BP: per-workgroup messages (and q for damped flooding, APP when
+    // it does not fit LDS)
+    DevBuf<float> long_r, long_q, long_app;
+    void ensure_long(int blocks, int slots, int n, bool need_q, bool need_app);
+};
+
+// Device tables of the long-code BP kernel for one schedule.
+struct LongTables {
+    DevBuf<int> grp_row, grp_edge, grp_width, edge_col, col_ptr, col_slot;
+    BpLongCodeDev dev;
 };
 
 class Engine {
@@ -43,16 +54,23 @@ public:
     bool osd_supported() const { return osd_ok_; }
 
     // BP over num_cw device-resident codewords, then OSD (order >= 0) on the
-    // frames BP did not accept.  osd_order < 0 disables OSD.
+    // frames BP did not accept.  osd_order < 0 disables OSD.  The flooding
+    // schedule of a code the short-code tables hold runs the short-code kernel;
+    // layered schedules and longer codes run the long-code kernel (no OSD).
     void decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
                 const DecodeOut& out, hipStream_t s, DecodeScratch* scratch = nullptr);
     // OSD on every codeword (no BP).
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 
     int max_waves() const { return max_waves_; }
-    // Pre-size the OSD hand-off scratch for batches of up to num_cw codewords
-    // (decode() grows it on demand, which a graph capture cannot do).
-    void reserve(int num_cw) { own_.ensure(num_cw, code_.n); }
+    // Pre-size the OSD hand-off scratch for batches of up to num_cw codewords,
+    // and the long-code scratch when `bp` selects that kernel (decode() grows
+    // them on demand, which a graph capture cannot do).
+    void reserve(int num_cw, const BpParams* bp = nullptr);
+    // True when `bp` runs the long-code kernel (layered schedule or a code the
+    // short-code tables cannot hold).
+    bool uses_long_kernel(const BpParams& bp) const;
+    bool short_kernel_ok() const { return short_ok_; }
 
     // BP launch shape.  autotune() times every supported shape (wavefronts per
     // codeword, codewords per workgroup, persistent grid size) on a sample batch
@@ -86,8 +104,16 @@ private:
 
 private:
 
+    void decode_long(const float* d_llr, int num_cw, const BpParams& bp, bool layered, const DecodeOut& out,
+                     hipStream_t s, DecodeScratch& sc);
+    const LongTables& long_tables(bool layered);
+    BpLongScratch long_scratch(const LongTables& t, const BpParams& bp, bool layered, DecodeScratch& sc);
+
     LdpcCode code_;
+    bool short_ok_ = false;
     bool osd_ok_ = false;
+    int cus_ = 256;
+    std::unique_ptr<LongTables> long_[2];  // [0] flooding, [1] layered
     int max_waves_ = 8192;
     DevBuf<int> row_meta_;
     DevBuf<int> col_meta_;

@@ -31,6 +31,9 @@ struct BpParams {
     float offset = 0.0f;     // magnitude offset, mag = max(mag - offset, 0)
     float damping = 0.0f;    // VN message damping (MATLAB reference option)
     float min_abs_accept = 0.0f;  // a converged frame needs min|APP| >= this
+    // 0 flooding, 1 node-parallel (= flooding), 2 colored layered, 3 flat layered
+    // (= colored layered), < 0 automatic: layered when n >= 1000.
+    int schedule = 0;
 };
 
 // Per-launch inputs/outputs of the BP kernel.  Null outputs are skipped.
@@ -110,6 +113,42 @@ bool bp_team_supported(const BpCodeDev& c, int team);
 int bp_default_team(const BpCodeDev& c);
 void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves,
                      hipStream_t s, const BpTune& tune = BpTune());
+// Long-code BP (flooding or layered), one workgroup of 16 wavefronts per
+// codeword (8 for row degrees above 16).
+// Rows are processed in groups (layered: colours of the greedy row colouring;
+// flooding: one group of all rows in row order).  Group g covers positions
+// [grp_row[g], grp_row[g + 1]) of the processing order; edge j of the row at
+// position p of the group is slot grp_edge[g] + j * rows_g + p, with
+// edge_col[slot] = column, or -1 past the row's degree.
+struct BpLongCodeDev {
+    int n = 0;
+    int m = 0;
+    int k = 0;
+    int slots = 0;  // padded edge slots
+    int groups = 0;
+    int max_width = 0;
+    int layered = 0;
+    const int* grp_row = nullptr;    // [groups + 1]
+    const int* grp_edge = nullptr;   // [groups]
+    const int* grp_width = nullptr;  // [groups]
+    const int* edge_col = nullptr;   // [slots]
+    const int* col_ptr = nullptr;    // [n + 1] flooding: column v's slots, increasing row
+    const int* col_slot = nullptr;   // [edges]
+    const int* info_cols = nullptr;  // [k]
+};
+// Per-workgroup scratch: r (and q for damped flooding) [blocks][slots]; APP
+// [blocks][n] unless it is kept in LDS.
+struct BpLongScratch {
+    float* r = nullptr;
+    float* q = nullptr;
+    float* app = nullptr;
+    int blocks = 0;
+    bool app_in_lds = false;
+};
+bool bp_long_app_fits_lds(int n);
+void launch_bp_long(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, const BpLongScratch& sc,
+                    hipStream_t s);
+
 bool osd_shape_supported(int n, int k);
 void launch_osd(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames,
                 hipStream_t s);

@@ -225,9 +225,23 @@ public:
         if (lat) *lat = static_cast<double>(j->t_done - j->t_submit);
     }
 
+    // While held, arrivals queue and nothing new is admitted or launched
+    // (launches in flight still complete); releasing lets the scheduler pack
+    // the queued burst.
+    void hold(bool on)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            hold_ = on;
+        }
+        cv_.notify_all();
+    }
+
     void drain()
     {
         std::unique_lock<std::mutex> lk(mu_);
+        if (hold_ && st_.jobs_completed != st_.jobs_submitted)
+            throw std::invalid_argument("drain: the server is held with jobs pending");
         cv_done_.wait(lk, [&] { return st_.jobs_completed == st_.jobs_submitted || failed_; });
         if (failed_) throw std::runtime_error("server: " + fail_msg_);
     }
@@ -268,16 +282,18 @@ private:
             std::unique_lock<std::mutex> lk(mu_);
             while (!stop_) {
                 bool progress = harvest();
-                progress |= admit();
-                progress |= launch();
+                if (!hold_) {
+                    progress |= admit();
+                    progress |= launch();
+                }
                 const bool inflight = std::any_of(slots_.begin(), slots_.end(), [](const Slot& s) { return s.busy; });
                 if (!progress) {
-                    if (inflight || !arrivals_.empty() || !resident_.empty()) {
+                    if (inflight || (!hold_ && (!arrivals_.empty() || !resident_.empty()))) {
                         lk.unlock();
                         std::this_thread::yield();
                         lk.lock();
                     } else {
-                        cv_.wait(lk, [&] { return stop_ || !arrivals_.empty(); });
+                        cv_.wait(lk, [&] { return stop_ || (!hold_ && (!arrivals_.empty() || !resident_.empty())); });
                     }
                 }
             }
@@ -437,6 +453,7 @@ private:
     std::unordered_map<uint64_t, std::shared_ptr<Job>> known_;
     sicc_server_stats_t st_{};
     bool stop_ = false;
+    bool hold_ = false;
     bool failed_ = false;
     std::string fail_msg_;
     std::thread thread_;
@@ -522,6 +539,14 @@ int sicc_server_drain(unsigned long long handle)
 {
     return guarded([&] {
         slookup(handle)->drain();
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_hold(unsigned long long handle, int hold)
+{
+    return guarded([&] {
+        slookup(handle)->hold(hold != 0);
         return CUDA_OSD_STATUS_OK;
     });
 }

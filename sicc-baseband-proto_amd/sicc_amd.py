@@ -412,6 +412,10 @@ class Server:
     def drain(self):
         check(self.lib.sicc_server_drain(self.handle))
 
+    def hold(self, on: bool):
+        self.lib.sicc_server_hold.argtypes = [C.c_ulonglong, C.c_int]
+        check(self.lib.sicc_server_hold(self.handle, int(on)))
+
     def drive(self, members, counts, d_ptrs, arrivals_us):
         n = len(members)
         mem = np.ascontiguousarray(members, np.int32)

@@ -132,7 +132,7 @@ def test_bp_csr_decoder_other_codes():
     for (n, m, rp, ci), ebn0 in [((n, m, rp, ci), 2.0), ((1024, m_qc, rp_qc, ci_qc), 1.8)]:
         code = oracle.Code(n, m, rp, ci, front_info=True)
         _, _, llr = _frames(code, 300, ebn0, n)
-        with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75) as dec:
+        with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0) as dec:
             bits, ok, _ = dec.bp_csr_batch(llr)
         for i in range(llr.shape[0]):
             o = code.nms(llr[i], max_iter=30, alpha=0.75)
@@ -161,7 +161,7 @@ def test_every_launch_shape_bit_exact(which):
     _, _, llr = _frames(code, 96, 1.5 if n <= 256 else 2.0, 3 * n + m)
     ref = [code.nms(llr[i], max_iter=25, alpha=0.75) for i in range(llr.shape[0])]
     d_llr = torch.from_numpy(llr).cuda()
-    with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=25, alpha=0.75) as dec:
+    with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=25, alpha=0.75, schedule=0) as dec:
         for team in (1, 2, 4, 8, 16):
             for tpb in ((1, 2, 4) if team == 1 else (1,)):
                 try:

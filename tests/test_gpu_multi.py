@@ -23,7 +23,7 @@ def _setup(counts, seed):
         code = oracle.Code(n, m, rp, ci, front_info=True)
         info = rng.integers(0, 2, (cnt, code.k), dtype=np.uint8)
         llr = oracle.awgn_llr(code.encode(info), 2.2, code.k / n, rng)
-        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75))
+        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0))
         codes.append(code)
         d = {"llr": llr, "d_llr": torch.from_numpy(llr).cuda(),
              "d_info": torch.zeros((cnt, code.k), dtype=torch.uint8, device="cuda"),

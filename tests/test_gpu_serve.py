@@ -20,7 +20,7 @@ def _members():
     codes.append(oracle.Code(n_, m_, rp, ci, True))
     for n, mb in [(256, 8), (1024, 4), (2048, 8)]:
         m, rp, ci = sa.qc_code(n, mb, 3, seed=n + mb)
-        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75))
+        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0))
         codes.append(oracle.Code(n, m, rp, ci, True))
     return decs, codes
 
@@ -54,8 +54,12 @@ def test_alone_equals_in_mix():
             srv.submit(1000 + j, mem, llr)
             srv.drain()
             alone[j] = srv.fetch(1000 + j, llr.shape[0], codes[mem].k)
-        for j, (mem, llr) in enumerate(jobs):  # all at once: packed, overlapped launches
+        srv.hold(True)  # queue the burst, then release it: packed, overlapped launches
+        for j, (mem, llr) in enumerate(jobs):
             srv.submit(5000 + j, mem, llr)
+        with pytest.raises(sa.SiccError, match="held"):
+            srv.drain()
+        srv.hold(False)
         srv.drain()
         done = srv.poll(4096)
         assert set(range(5000, 5060)) <= set(done)
