@@ -16,7 +16,9 @@ in HBM (one BP launch + one OSD launch).  Alongside, in the same line:
            CFO 0.3 bin, 0..200 sample offset, 14 dB) synced, demodulated and
            decoded with a rate-1/2 n=1024 QC-LDPC code (30 iterations);
   osd      262 144 (128,64) frames at Eb/N0 1.0..2.5 dB, 20 BP iterations and
-           order-2 OSD (2 081 test patterns) on every BP failure.
+           order-2 OSD (2 081 test patterns) on every BP failure;
+  long     DVB-S2 N16200 and N64800 rate-1/2 codes, layered schedule, the
+           reference's CSR-decoder defaults, 50 iterations.
 Every figure is measured on data resident in HBM; host transfers are outside
 the timed regions.
 """
@@ -386,6 +388,58 @@ def leg_osd(a, sa, np, torch, stream):
     return {"frames_per_point": F, "bp_iterations": 20, "osd_order": 2, "teps": 2081, "points": pts}
 
 
+# ------------------------------------------------------------------ long ----
+def leg_long(a, sa, np, torch, stream):
+    """DVB-S2 codes on the long-code kernel, layered schedule, with the reference's
+    CSR-decoder defaults (normalization 0.80, offset 0.15, damping 0.15)."""
+    import oracle
+
+    dev = torch.device("cuda:0")
+    sh = stream.cuda_stream
+    rng = np.random.default_rng(61)
+    pts = []
+    for name, ebn0, F in (("DVB_S2_short_N16200_rate_1_2.alist", 2.4, a.long_frames),
+                          ("DVB_S2_N64800_R12.alist", 1.9, max(a.long_frames // 4, 1))):
+        n, m, rp, ci = oracle.alist_parse(os.path.join(sa.DATA_DIR, name))
+        code = oracle.Code(n, m, rp, ci, True)
+        k = code.k
+        info = rng.integers(0, 2, (F, k), dtype=np.uint8)
+        cw = code.encode(info)
+        llr = oracle.awgn_llr(cw, ebn0, k / n, rng)
+        d_llr = torch.from_numpy(llr).to(dev)
+        del llr, cw
+        d_info = torch.empty((F, k), dtype=torch.uint8, device=dev)
+        d_ok = torch.empty(F, dtype=torch.uint8, device=dev)
+        d_it = torch.empty(F, dtype=torch.int32, device=dev)
+        dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=50, alpha=0.8, offset=0.15, damping=0.15, schedule=2)
+        torch.cuda.synchronize()
+
+        def run():
+            dec.decode_device(d_llr.data_ptr(), F, d_info=d_info.data_ptr(), d_ok=d_ok.data_ptr(),
+                              d_iters=d_it.data_ptr(), stream=sh)
+
+        run()
+        torch.cuda.synchronize()
+        dec.set_timing(True)
+        ms = timed(run, a.long_reps, stream)
+        kt = dec.timing()
+        dec.set_timing(False)
+        kms = kt["bp_ms"] / max(kt["bp_launches"], 1)
+        iters = d_it.cpu().numpy()
+        fe = int((d_info.cpu().numpy() != info).any(axis=1).sum())
+        E = int(rp[-1])
+        pts.append({"code": name.replace(".alist", ""), "n": n, "k": k, "edges": E, "frames": F, "ebn0_db": ebn0,
+                    "schedule": "layered (greedy colouring)", "alpha": 0.8, "offset": 0.15, "damping": 0.15,
+                    "fer": round(fe / F, 5), "mean_iterations": round(float(iters.mean()), 2),
+                    "ms": round(ms, 3), "kernel_ms": round(kms, 3),
+                    "value": round(F * k / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
+                    "codewords_per_s": round(F / (ms * 1e-3), 1),
+                    "edge_updates_per_s": round(float(iters.sum()) * E / (kms * 1e-3), 1)})
+        dec.close()
+        del d_llr, d_info
+    return {"points": pts}
+
+
 # ------------------------------------------------------------------- cpu ----
 def cpu_baselines(a, sa, np, llr, got):
     import oracle
@@ -435,8 +489,10 @@ def main() -> int:
     ap.add_argument("--ofdm-frames", type=int, default=4096)
     ap.add_argument("--ofdm-reps", type=int, default=5)
     ap.add_argument("--osd-frames", type=int, default=262144)
-    ap.add_argument("--legs", default="served,mixed,ofdm,osd,cpu",
-                    help="comma list of extra legs: served, mixed, ofdm, osd, cpu ('' = headline only)")
+    ap.add_argument("--long-frames", type=int, default=4096)
+    ap.add_argument("--long-reps", type=int, default=3)
+    ap.add_argument("--legs", default="served,mixed,ofdm,osd,long,cpu",
+                    help="comma list of extra legs: served, mixed, ofdm, osd, long, cpu ('' = headline only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one device copy of the LLR buffer (known bytes) to scale HBM counters")
@@ -474,6 +530,9 @@ def main() -> int:
     if "osd" in legs:
         log("osd")
         extra["osd"] = leg_osd(a, sa, np, torch, stream)
+    if "long" in legs:
+        log("long")
+        extra["long"] = leg_long(a, sa, np, torch, stream)
     cpu, cpu_ref = (None, None)
     if "cpu" in legs:
         log("cpu")
