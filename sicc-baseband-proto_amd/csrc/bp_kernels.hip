@@ -102,10 +102,13 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
         }
     }
     int cm[NPL];
+    unsigned uni = 0u;  // bit t: all lanes of this wavefront hold a variable of one degree
 #pragma unroll
     for (int t = 0; t < NPL; ++t) {
         const int v = tid + T * t;
         cm[t] = v < n ? code.col_meta[v] : 0;
+        const int dg0 = __shfl(cm[t] & 255, 0);
+        if (__ballot(v >= n || (cm[t] & 255) != dg0) == 0ull) uni |= 1u << t;
     }
     if (tid == 0) {
         app[np] = FLT_MAX;
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
             lr[t] = v < n ? L[v] : 0.0f;
             appr[t] = lr[t];
             bestr[t] = lr[t];
-            app[v] = lr[t];
+            app[v] = lr[t] + 0.0f;  // LDS copy canonical (-0 -> +0): its sign bit is (APP < 0)
         }
         float rold[MPL][MAXDC];
         float qold[MPL][MAXDC];
@@ -173,7 +176,9 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
                     const float a = app[em[s][j] >> 16];
-                    hp ^= __float_as_uint(a + 0.0f);  // +0.0f maps -0 to +0: sign bit == (a < 0)
+                    hp ^= __float_as_uint(a);  // canonical APP: sign bit == (APP < 0)
+                    // q differs from the exact APP's only in the sign of a zero, which
+                    // the canonicalization below removes
                     float q = a - rold[s][j];
                     if (DAMP) {
                         if (it > 0) q = (1.0f - damp) * q + damp * qold[s][j];
@@ -183,7 +188,8 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                     qv[j] = q;
                     sp ^= __float_as_uint(q);
                     const float av = fabsf(q);
-                    m2 = fminf(m2, fmaxf(m1, av));
+                    // m1 <= m2 always, so min(m2, max(m1, av)) is the median of the three
+                    m2 = __builtin_amdgcn_fmed3f(m1, m2, av);
                     m1 = fminf(m1, av);
                 }
                 synw += __popcll(__ballot((hp >> 31) != 0u));
@@ -196,6 +202,8 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                 }
                 a1 = alpha * a1;
                 a2 = alpha * a2;
+                // keep the two products per row (not one per edge after the select)
+                asm volatile("" : "+v"(a1), "+v"(a2));
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
                     const float mag = (fabsf(qv[j]) == m1) ? a2 : a1;
@@ -231,13 +239,22 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
 #pragma unroll
             for (int t = 0; t < NPL; ++t) {
                 const int v = tid + T * t;
-                if (v < n) {
+                const int st = cm[t] >> 8;
+                if (uni & (1u << t)) {
+                    // every lane of the wavefront has this degree: scalar trip count
+                    const int dgs = __builtin_amdgcn_readfirstlane(cm[t] & 255);
                     float a = lr[t];
-                    const int st = cm[t] >> 8;
+                    const float* rc = rcol + st;
+#pragma unroll 4
+                    for (int j = 0; j < dgs; ++j) a += rc[j];
+                    appr[t] = a;
+                    app[v] = a + 0.0f;
+                } else if (v < n) {
+                    float a = lr[t];
                     const int dg = cm[t] & 255;
                     for (int j = 0; j < dg; ++j) a += rcol[st + j];
                     appr[t] = a;
-                    app[v] = a;
+                    app[v] = a + 0.0f;
                 }
             }
             team_sync<TEAM>();

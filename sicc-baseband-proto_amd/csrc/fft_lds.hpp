@@ -18,39 +18,48 @@ static __device__ void fft_radix4(Cd* buf, const double2* tw, int N)
 {
     const int q = N >> 2;
     for (int ns = 1; ns < N; ns <<= 2) {
-        Cd v[4][2];
+        // up to two butterflies per thread (N <= 2048 with 256 threads); fixed
+        // indices keep the staged outputs in registers
+        Cd v[2][4];
         int outi[2];
-        int cnt = 0;
-        for (int j = threadIdx.x; j < q; j += blockDim.x, ++cnt) {
-            const int k = j & (ns - 1);
-            Cd a0 = buf[j], a1 = buf[j + q], a2 = buf[j + 2 * q], a3 = buf[j + 3 * q];
-            if (ns > 1) {
-                // twiddle w^(r k), w = exp(-2 pi i / (4 ns)); table entry m = exp(-2 pi i m / N)
-                const int step = N / (4 * ns);
-                const double2 t1 = tw[(k * step) & (N - 1)];
-                const double2 t2 = tw[(2 * k * step) & (N - 1)];
-                const double2 t3 = tw[(3 * k * step) & (N - 1)];
-                a1 = cmul(a1, {t1.x, t1.y});
-                a2 = cmul(a2, {t2.x, t2.y});
-                a3 = cmul(a3, {t3.x, t3.y});
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int j = threadIdx.x + c * blockDim.x;
+            outi[c] = -1;
+            if (j < q) {
+                const int k = j & (ns - 1);
+                Cd a0 = buf[j], a1 = buf[j + q], a2 = buf[j + 2 * q], a3 = buf[j + 3 * q];
+                if (ns > 1) {
+                    // twiddle w^(r k), w = exp(-2 pi i / (4 ns)); table entry m = exp(-2 pi i m / N)
+                    const int step = N / (4 * ns);
+                    const double2 t1 = tw[(k * step) & (N - 1)];
+                    const double2 t2 = tw[(2 * k * step) & (N - 1)];
+                    const double2 t3 = tw[(3 * k * step) & (N - 1)];
+                    a1 = cmul(a1, {t1.x, t1.y});
+                    a2 = cmul(a2, {t2.x, t2.y});
+                    a3 = cmul(a3, {t3.x, t3.y});
+                }
+                const Cd b0 = {a0.x + a2.x, a0.y + a2.y};
+                const Cd b1 = {a0.x - a2.x, a0.y - a2.y};
+                const Cd b2 = {a1.x + a3.x, a1.y + a3.y};
+                const Cd b3 = {a1.y - a3.y, -(a1.x - a3.x)};  // (a1 - a3) * (-i)
+                v[c][0] = {b0.x + b2.x, b0.y + b2.y};
+                v[c][2] = {b0.x - b2.x, b0.y - b2.y};
+                v[c][1] = {b1.x + b3.x, b1.y + b3.y};
+                v[c][3] = {b1.x - b3.x, b1.y - b3.y};
+                outi[c] = (j / ns) * 4 * ns + k;
             }
-            const Cd b0 = {a0.x + a2.x, a0.y + a2.y};
-            const Cd b1 = {a0.x - a2.x, a0.y - a2.y};
-            const Cd b2 = {a1.x + a3.x, a1.y + a3.y};
-            const Cd b3 = {a1.y - a3.y, -(a1.x - a3.x)};  // (a1 - a3) * (-i)
-            v[0][cnt] = {b0.x + b2.x, b0.y + b2.y};
-            v[2][cnt] = {b0.x - b2.x, b0.y - b2.y};
-            v[1][cnt] = {b1.x + b3.x, b1.y + b3.y};
-            v[3][cnt] = {b1.x - b3.x, b1.y - b3.y};
-            outi[cnt] = (j / ns) * 4 * ns + k;
         }
         __syncthreads();
-        for (int c = 0; c < cnt; ++c) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
             const int o = outi[c];
-            buf[o] = v[0][c];
-            buf[o + ns] = v[1][c];
-            buf[o + 2 * ns] = v[2][c];
-            buf[o + 3 * ns] = v[3][c];
+            if (o >= 0) {
+                buf[o] = v[c][0];
+                buf[o + ns] = v[c][1];
+                buf[o + 2 * ns] = v[c][2];
+                buf[o + 3 * ns] = v[c][3];
+            }
         }
         __syncthreads();
     }
