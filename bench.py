@@ -40,17 +40,20 @@ for _p in (PKG, os.path.join(ROOT, "oracle")):
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # CUs x bytes/clk x GHz
 
-# Eb/N0 (dB) per (n, block rows) of the mixed-code workload, chosen for a frame
-# error rate near 1e-2 with 50 min-sum iterations.
+# Eb/N0 (dB) per (n, block rows) of the mixed-code workload where the frame error
+# rate of 50-iteration flooding min-sum (alpha 0.75) crosses 1e-2 (mixed leg) and
+# 1e-3 (served mix), measured with `bench.py --calibrate-mix 40000` (40 000 frames
+# per 0.25 dB point, log-linear interpolation).
 MIX_EBN0 = {
-    (128, 8): 4.0, (256, 8): 3.4, (512, 8): 2.9, (1024, 8): 2.5, (2048, 8): 2.2,
-    (128, 5): 4.6, (256, 5): 4.0, (512, 5): 3.5, (1024, 5): 3.1, (2048, 5): 2.8,
-    (128, 4): 5.0, (256, 4): 4.4, (512, 4): 3.9, (1024, 4): 3.5, (2048, 4): 3.2,
+    (128, 8): 3.53, (256, 8): 3.05, (512, 8): 2.53, (1024, 8): 2.72, (2048, 8): 2.03,
+    (128, 5): 4.06, (256, 5): 3.55, (512, 5): 3.2, (1024, 5): 2.85, (2048, 5): 2.57,
+    (128, 4): 4.19, (256, 4): 3.84, (512, 4): 3.6, (1024, 4): 3.17, (2048, 4): 2.93,
 }
-
-
-# Eb/N0 (dB) per (n, block rows) for the served mix, frame error rate near 1e-3.
-MIX_EBN0_SERVE = {key: e + 0.5 for key, e in MIX_EBN0.items()}
+MIX_EBN0_SERVE = {
+    (128, 8): 4.19, (256, 8): 3.59, (512, 8): 2.96, (1024, 8): 3.0, (2048, 8): 2.22,
+    (128, 5): 4.84, (256, 5): 4.07, (512, 5): 3.62, (1024, 5): 3.14, (2048, 5): 2.78,
+    (128, 4): 4.89, (256, 4): 4.41, (512, 4): 4.4, (1024, 4): 3.45, (2048, 4): 3.12,
+}
 
 
 def log(msg):

@@ -140,13 +140,42 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
         cw = b;
         lim = min(b + chunk, io.num_cw);
     }
-    while (cw < io.num_cw) {
+    // information column of bit tid + T * t, and the LLRs of the next codeword,
+    // loaded while the current one decodes
+    int icol[NPL];
+    float lrn[NPL];
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) {
+        const int i = tid + T * t;
+        icol[t] = i < code.k ? code.info_cols[i] : 0;
+        lrn[t] = 0.0f;
+    }
+    if (cw < io.num_cw) {
         const float* L = io.llr + static_cast<size_t>(cw) * n;
-        float lr[NPL], appr[NPL], bestr[NPL];
 #pragma unroll
         for (int t = 0; t < NPL; ++t) {
             const int v = tid + T * t;
-            lr[t] = v < n ? L[v] : 0.0f;
+            lrn[t] = v < n ? L[v] : 0.0f;
+        }
+    }
+    while (cw < io.num_cw) {
+        float lr[NPL], appr[NPL], bestr[NPL];
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) lr[t] = lrn[t];
+        {
+            const int nx = steal ? (cw + 1 < lim ? cw + 1 : io.num_cw) : cw + nteams;
+            if (nx < io.num_cw) {
+                const float* Ln = io.llr + static_cast<size_t>(nx) * n;
+#pragma unroll
+                for (int t = 0; t < NPL; ++t) {
+                    const int v = tid + T * t;
+                    lrn[t] = v < n ? Ln[v] : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) {
+            const int v = tid + T * t;
             appr[t] = lr[t];
             bestr[t] = lr[t];
             app[v] = lr[t] + 0.0f;  // LDS copy canonical (-0 -> +0): its sign bit is (APP < 0)
@@ -295,8 +324,11 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
         }
         if (io.info_bits) {
             const int k = code.k;
-            for (int i = tid; i < k; i += T)
-                io.info_bits[static_cast<size_t>(cw) * k + i] = app[code.info_cols[i]] < 0.0f ? 1 : 0;
+#pragma unroll
+            for (int t = 0; t < NPL; ++t) {
+                const int i = tid + T * t;
+                if (i < k) io.info_bits[static_cast<size_t>(cw) * k + i] = app[icol[t]] < 0.0f ? 1 : 0;
+            }
         }
         if (tid == 0) {
             if (io.success) io.success[cw] = accept ? 1 : 0;
@@ -339,6 +371,14 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
             }
             cw = b;
             lim = min(b + chunk, io.num_cw);
+            if (cw < io.num_cw) {
+                const float* L = io.llr + static_cast<size_t>(cw) * n;
+#pragma unroll
+                for (int t = 0; t < NPL; ++t) {
+                    const int v = tid + T * t;
+                    lrn[t] = v < n ? L[v] : 0.0f;
+                }
+            }
         }
     }
 }
@@ -357,15 +397,21 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     while (tpb > 1 && team_bytes * tpb > 64 * 1024) --tpb;
     if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
     const size_t lds = team_bytes * tpb;
-    int teams = io.num_cw < max_waves / TEAM ? io.num_cw : max_waves / TEAM;
+    auto kern = damp ? bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>;
+    // A persistent grid larger than what the GPU holds at once would leave its
+    // last teams to start after the first ones finished all their work.
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), T * tpb, lds) !=
+            hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        throw std::runtime_error("bp_flood_kernel: occupancy query failed");
+    const long resident_teams = static_cast<long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1) * tpb;
+    long teams = io.num_cw < max_waves / TEAM ? io.num_cw : max_waves / TEAM;
+    if (teams > resident_teams) teams = resident_teams;
     if (teams < 1) teams = 1;
-    const int blocks = (teams + tpb - 1) / tpb;
-    if (damp)
-        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM>), dim3(blocks), dim3(T * tpb), lds, s, code,
-                           p, io, e_pad);
-    else
-        hipLaunchKernelGGL((bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>), dim3(blocks), dim3(T * tpb), lds, s, code,
-                           p, io, e_pad);
+    const int blocks = static_cast<int>((teams + tpb - 1) / tpb);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, io, e_pad);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("bp_flood_kernel launch: ") + hipGetErrorString(e));
 }
