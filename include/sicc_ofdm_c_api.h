@@ -63,7 +63,9 @@ int sicc_ofdm_create(const sicc_ofdm_config_t* config, const float* preamble_iq,
 int sicc_ofdm_destroy(unsigned long long handle);
 int sicc_ofdm_llr_per_frame(unsigned long long handle);
 
-/* Device-resident batches on `hip_stream` (null: the receiver's own stream), asynchronous. */
+/* Device-resident batches on `hip_stream` (null: the receiver's own stream), asynchronous.
+ * Demodulation keeps per-frame channel estimates in receiver-owned scratch:
+ * calls on one handle must be issued on one stream (or must not overlap). */
 int sicc_ofdm_sync_device(unsigned long long handle, const float* d_rx_iq, int frames, int span,
                           sicc_ofdm_sync_t* d_sync, void* hip_stream);
 int sicc_ofdm_demod_device(unsigned long long handle, const float* d_rx_iq, int frames, int span,

@@ -195,13 +195,15 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
         bool ok = false;
         int it = 0;
         for (;;) {
-            // ---- check-node update; its APP gather also yields the syndrome ----
+            // ---- check-node gather: q, min1 / min2 / sign, and the syndrome ----
             int synw = 0;
+            float qv[MPL][MAXDC];
+            float m1s[MPL], m2s[MPL];
+            unsigned sps[MPL];
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
                 unsigned hp = 0u, sp = 0u;
                 float m1 = FLT_MAX, m2 = FLT_MAX;
-                float qv[MAXDC];
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
                     const float a = app[em[s][j] >> 16];
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                         qold[s][j] = q;
                     }
                     q = q + 0.0f;
-                    qv[j] = q;
+                    qv[s][j] = q;
                     sp ^= __float_as_uint(q);
                     const float av = fabsf(q);
                     // m1 <= m2 always, so min(m2, max(m1, av)) is the median of the three
@@ -222,24 +224,9 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                     m1 = fminf(m1, av);
                 }
                 synw += __popcll(__ballot((hp >> 31) != 0u));
-                // The first minimum edge takes min2; any other edge equal to min1
-                // implies min2 == min1, so |q| == min1 selects the same magnitude.
-                float a1 = m1, a2 = m2;
-                if (off != 0.0f) {
-                    a1 = fmaxf(a1 - off, 0.0f);
-                    a2 = fmaxf(a2 - off, 0.0f);
-                }
-                a1 = alpha * a1;
-                a2 = alpha * a2;
-                // keep the two products per row (not one per edge after the select)
-                asm volatile("" : "+v"(a1), "+v"(a2));
-#pragma unroll
-                for (int j = 0; j < MAXDC; ++j) {
-                    const float mag = (fabsf(qv[j]) == m1) ? a2 : a1;
-                    const float r = __uint_as_float(__float_as_uint(mag) | ((sp ^ __float_as_uint(qv[j])) & 0x80000000u));
-                    rold[s][j] = r;
-                    rcol[em[s][j] & 0xffff] = r;
-                }
+                m1s[s] = m1;
+                m2s[s] = m2;
+                sps[s] = sp;
             }
             if (TEAM > 1) {
                 // team-wide syndrome weight through an LDS counter, double-buffered by iteration
@@ -257,12 +244,36 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                 }
                 if (synw == 0) {
                     ok = true;
-                    break;
+                    break;  // the messages of this pass would never be used
                 }
             } else {
                 best_w = synw;
             }
             if (it == p.max_iter) break;
+            // ---- check-node messages ----
+#pragma unroll
+            for (int s = 0; s < MPL; ++s) {
+                // The first minimum edge takes min2; any other edge equal to min1
+                // implies min2 == min1, so |q| == min1 selects the same magnitude.
+                const float m1 = m1s[s];
+                float a1 = m1, a2 = m2s[s];
+                if (off != 0.0f) {
+                    a1 = fmaxf(a1 - off, 0.0f);
+                    a2 = fmaxf(a2 - off, 0.0f);
+                }
+                a1 = alpha * a1;
+                a2 = alpha * a2;
+                // keep the two products per row (not one per edge after the select)
+                asm volatile("" : "+v"(a1), "+v"(a2));
+#pragma unroll
+                for (int j = 0; j < MAXDC; ++j) {
+                    const float mag = (fabsf(qv[s][j]) == m1) ? a2 : a1;
+                    const float r =
+                        __uint_as_float(__float_as_uint(mag) | ((sps[s] ^ __float_as_uint(qv[s][j])) & 0x80000000u));
+                    rold[s][j] = r;
+                    rcol[em[s][j] & 0xffff] = r;
+                }
+            }
             team_sync<TEAM>();
             // ---- variable-node update: APP = LLR + sum r, increasing row order ----
 #pragma unroll

@@ -85,7 +85,7 @@ unsigned long long make_stream(const cuda_osd_stream_config_t* cfg, LdpcCode cod
     d->kind = Kind::Stream;
     init_common(*d);
     d->eng = std::make_unique<Engine>(std::move(code));
-    if (!d->eng->osd_supported()) throw std::invalid_argument("OSD supports k <= 64 and n <= 256");
+    if (!d->eng->osd_supported()) throw std::invalid_argument("OSD supports k <= 128 and n <= 256");
     d->cfg = *cfg;
     d->bp.max_iter = cfg->bp_max_iterations;
     d->bp.alpha = cfg->bp_normalization;
@@ -394,7 +394,7 @@ int cuda_osd_create_osd_decoder(const cuda_osd_osd_config_t* config, unsigned lo
         d->kind = Kind::OsdOnly;
         init_common(*d);
         d->eng = std::make_unique<Engine>(bundled_ccsds(config->n, config->k));
-        if (!d->eng->osd_supported()) throw std::invalid_argument("OSD supports k <= 64 and n <= 256");
+        if (!d->eng->osd_supported()) throw std::invalid_argument("OSD supports k <= 128 and n <= 256");
         d->osd.order = 2;
         d->osd_order = 2;
         d->osd.max_flip = config->k;

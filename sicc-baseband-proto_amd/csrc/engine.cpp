@@ -69,14 +69,17 @@ Engine::Engine(LdpcCode code) : code_(std::move(code))
     osd_ok_ = short_ok_ && osd_shape_supported(n, code_.k);
     if (osd_ok_) {
         const auto g = code_.dense_generator();
-        std::vector<unsigned long long> cols(n, 0);
+        const int kw = (code_.k + 63) / 64;
+        std::vector<unsigned long long> cols(static_cast<size_t>(n) * kw, 0);
         for (int r = 0; r < code_.k; ++r)
             for (int c = 0; c < n; ++c)
-                if (g[static_cast<size_t>(r) * n + c]) cols[c] |= 1ull << r;
-        gen_cols_.alloc(n);
-        SICC_HIP(hipMemcpy(gen_cols_.get(), cols.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice));
+                if (g[static_cast<size_t>(r) * n + c]) cols[static_cast<size_t>(c) * kw + r / 64] |= 1ull << (r % 64);
+        gen_cols_.alloc(cols.size());
+        SICC_HIP(hipMemcpy(gen_cols_.get(), cols.data(), cols.size() * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice));
         osddev_.n = n;
         osddev_.k = code_.k;
+        osddev_.kw = kw;
         osddev_.gen_cols = gen_cols_.get();
         osddev_.info_cols = info_cols_.get();
     }
@@ -271,7 +274,7 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         decode_long(d_llr, num_cw, bp, bp.schedule >= 2 || (bp.schedule < 0 && code_.n >= 1000), out, s, sc);
         return;
     }
-    if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 64, n <= 256)");
+    if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 128, n <= 256)");
     BpIo io;
     io.llr = d_llr;
     io.num_cw = num_cw;
@@ -318,7 +321,7 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
 void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s)
 {
     if (num_cw <= 0) return;
-    if (!osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 64, n <= 256)");
+    if (!osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 128, n <= 256)");
     OsdIo oio;
     oio.llr = d_llr;
     oio.host_count = num_cw;

@@ -57,12 +57,13 @@ struct BpIo {
     int chunk = 1;
 };
 
-// OSD inputs: the systematic generator as column words (bit r of gen_cols[c]
-// is G[r][c], k <= 64) and the frames BP handed off.
+// OSD inputs: the generator as column words (bit r % 64 of gen_cols[c * kw + r / 64]
+// is G[r][c], k <= 128, kw = ceil(k / 64)) and the frames BP handed off.
 struct OsdCodeDev {
     int n = 0;
     int k = 0;
-    const unsigned long long* gen_cols = nullptr;  // [n]
+    int kw = 1;
+    const unsigned long long* gen_cols = nullptr;  // [n][kw]
     const int* info_cols = nullptr;                // [k]
 };
 

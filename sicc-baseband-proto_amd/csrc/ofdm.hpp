@@ -59,6 +59,7 @@ struct OfdmDemodArgs {
     int use_pilots = 1;
     float* llr = nullptr;  // [frames][llr_per_frame]
     int llr_per_frame = 0;
+    float2* h_est = nullptr;  // scratch [frames][n_pilot][active_sc] LS estimates
 };
 
 void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s);

@@ -221,6 +221,14 @@ void OfdmRx::demod(const float2* d_rx, int frames, int span, const OfdmSyncResul
     a.use_pilots = cfg_.use_pilot_channel_est;
     a.llr = d_llr;
     a.llr_per_frame = llr_per_frame();
+    if (cfg_.use_pilot_channel_est) {
+        const size_t need = static_cast<size_t>(frames) * n_pilot_ * cfg_.active_sc;
+        if (h_est_.size() < need) {
+            SICC_HIP(hipStreamSynchronize(s));  // an earlier demod on this stream may still read it
+            h_est_.alloc(need);
+        }
+        a.h_est = h_est_.get();
+    }
     launch_ofdm_demod(a, frames, s);
 }
 

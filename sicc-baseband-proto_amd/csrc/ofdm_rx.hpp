@@ -51,6 +51,9 @@ private:
     DevBuf<int> pilot_symbols_, data_symbols_, used_natural_, p0_, p1_;
     DevBuf<float> w0_, w1_;
     DevBuf<double2> twiddle_;
+    // LS channel estimates of the last demod batch; demods on one receiver
+    // must be issued on one stream (or not overlap)
+    mutable DevBuf<float2> h_est_;
 };
 
 // Gray QAM constellation indexed by the symbol's bits MSB first, unit average

@@ -20,11 +20,13 @@
 // Mapping.  Sync: one 256-thread workgroup per frame; the search window sits
 // in LDS, each thread sums whole candidates in FP64 (direct sums, no sliding
 // recurrence), then block argmax / refine / CFO.  Demod: one 256-thread
-// workgroup per frame walks its symbols: derotate + CP strip while loading the
-// FFT input, radix-4 Stockham FFT in FP64 through LDS (5 stages for 1024),
-// pilots first (LS estimates kept in LDS), then each data symbol is equalized
-// and turned into LLRs straight from LDS, so the frequency grid never goes to
-// HBM: per frame the kernel reads the symbol windows once and writes LLRs once.
+// workgroup per (frame, pilot symbol) writes LS estimates (A floats2 per pilot,
+// L2-resident), then one per (frame, data symbol) derotates + strips the CP
+// while loading the FFT input, runs the radix-4 Stockham FFT in FP64 through
+// LDS (5 stages for 1024) and equalizes and turns the symbol into LLRs straight
+// from LDS: the frequency grid never goes to HBM, each symbol window is read
+// once and each LLR written once, and a batch is tens of thousands of
+// independent workgroups.
 #include "fft_lds.hpp"
 #include "ofdm.hpp"
 
@@ -199,98 +201,123 @@ __global__ __launch_bounds__(kThreads) void ofdm_sync_kernel(OfdmSyncArgs a)
 }
 
 // --------------------------------------------------------------- demod ----
-template <int B>
-__global__ __launch_bounds__(kThreads) void ofdm_demod_kernel(OfdmDemodArgs a)
+__device__ __forceinline__ bool frame_usable(const OfdmDemodArgs& a, const OfdmSyncResult& sr)
+{
+    return sr.start >= 0 && static_cast<long long>(sr.start) + a.frame_samples <= a.span;
+}
+
+// Derotate (phase origin at the frame start), strip the CP and FFT symbol `sym`
+// of frame x into buf (natural bin order, FP64).
+__device__ void symbol_fft(const OfdmDemodArgs& a, const float2* x, double cfo, int sym, Cd* buf)
+{
+    const int N = a.nfft;
+    const int base = a.preamble_len + sym * (N + a.cp) + a.cp;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const int n = base + k;
+        const double ang = -2.0 * kPiD * cfo * static_cast<double>(n) / a.fs;
+        const float c = static_cast<float>(cos(ang));
+        const float s = static_cast<float>(sin(ang));
+        const float2 v = x[n];
+        const float re = v.x * c - v.y * s;
+        const float im = v.x * s + v.y * c;
+        buf[k] = {static_cast<double>(re), static_cast<double>(im)};
+    }
+    __syncthreads();
+    fft_radix4(buf, a.twiddle, N);
+}
+
+// One workgroup per (frame, pilot symbol): LS estimate y conj(p) / max(|p|^2, 1e-9)
+// of every active subcarrier into h_est[frame][pilot][active_sc].
+__global__ __launch_bounds__(kThreads) void ofdm_pilot_kernel(OfdmDemodArgs a)
 {
     extern __shared__ double2 sh[];
-    Cd* buf = reinterpret_cast<Cd*>(sh);                         // [nfft]
-    float2* hp = reinterpret_cast<float2*>(sh + a.nfft);        // [n_pilot][active_sc]
-    const int f = blockIdx.x;
+    Cd* buf = reinterpret_cast<Cd*>(sh);
+    const int f = blockIdx.x / a.n_pilot;
+    const int pi = blockIdx.x - f * a.n_pilot;
+    const OfdmSyncResult sr = a.sync[f];
+    if (!frame_usable(a, sr)) return;
+    const float2* x = a.rx + static_cast<size_t>(f) * a.span + sr.start;
+    symbol_fft(a, x, sr.cfo_hz_d, a.pilot_symbols[pi], buf);
+    const int A = a.active_sc;
+    float2* h = a.h_est + (static_cast<size_t>(f) * a.n_pilot + pi) * A;
+    for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
+        const Cd yv = buf[a.used_natural[sc]];
+        const float yr = static_cast<float>(yv.x), yi = static_cast<float>(yv.y);
+        const float2 p = a.pilot_grid[pi * A + sc];
+        float den = p.x * p.x + p.y * p.y;
+        if (den < 1e-9f) den = 1e-9f;
+        const float tr = yr * p.x - yi * (-p.y);
+        const float ti = yr * (-p.y) + yi * p.x;
+        h[sc] = make_float2(tr / den, ti / den);
+    }
+}
+
+// One workgroup per (frame, data symbol): FFT, interpolated channel, equalize,
+// max-log LLRs of the symbol's resource elements.
+template <int B>
+__global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
+{
+    extern __shared__ double2 sh[];
+    Cd* buf = reinterpret_cast<Cd*>(sh);
+    const int f = blockIdx.x / a.n_data;
+    const int di = blockIdx.x - f * a.n_data;
+    const int A = a.active_sc;
     const OfdmSyncResult sr = a.sync[f];
     float* out = a.llr + static_cast<size_t>(f) * a.llr_per_frame;
-    if (sr.start < 0 || static_cast<long long>(sr.start) + a.frame_samples > a.span) {
+    if (!frame_usable(a, sr)) {
         // no frame detected, or the frame runs past the row: zero LLRs (erasures)
-        for (int i = threadIdx.x; i < a.llr_per_frame; i += blockDim.x) out[i] = 0.0f;
+        const int lo = di * A * B;
+        const int hi = min(lo + A * B, a.llr_per_frame);
+        for (int i = lo + static_cast<int>(threadIdx.x); i < hi; i += blockDim.x) out[i] = 0.0f;
         return;
     }
+    if (di * A * B >= a.llr_per_frame) return;  // a symbol of pad resource elements only
     const float2* x = a.rx + static_cast<size_t>(f) * a.span + sr.start;
-    const int N = a.nfft;
-    const int A = a.active_sc;
+    symbol_fft(a, x, sr.cfo_hz_d, a.data_symbols[di], buf);
     constexpr int M = 1 << B;
-    const double cfo = sr.cfo_hz_d;
     const float nv = a.noise_var;
-    for (int step = 0; step < a.n_pilot + a.n_data; ++step) {
-        const bool pilot = step < a.n_pilot;
-        const int sym = pilot ? a.pilot_symbols[step] : a.data_symbols[step - a.n_pilot];
-        const int base = a.preamble_len + sym * (N + a.cp) + a.cp;
-        for (int k = threadIdx.x; k < N; k += blockDim.x) {
-            const int n = base + k;
-            const double ang = -2.0 * kPiD * cfo * static_cast<double>(n) / a.fs;
-            const float c = static_cast<float>(cos(ang));
-            const float s = static_cast<float>(sin(ang));
-            const float2 v = x[n];
-            const float re = v.x * c - v.y * s;
-            const float im = v.x * s + v.y * c;
-            buf[k] = {static_cast<double>(re), static_cast<double>(im)};
+    const int p0 = a.interp_p0[di], p1 = a.interp_p1[di];
+    const float w0 = a.interp_w0[di], w1 = a.interp_w1[di];
+    const float2* h0p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p0) * A : nullptr;
+    const float2* h1p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p1) * A : nullptr;
+    for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
+        const int obit = (di * A + sc) * B;
+        if (obit >= a.llr_per_frame) continue;  // pad resource elements
+        float hr = 1.0f, hi = 0.0f;
+        if (a.use_pilots) {
+            const float2 h0 = h0p[sc], h1 = h1p[sc];
+            hr = h0.x * w0 + h1.x * w1;
+            hi = h0.y * w0 + h1.y * w1;
         }
-        __syncthreads();
-        fft_radix4(buf, a.twiddle, N);
-        if (pilot) {
-            for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
-                const Cd yv = buf[a.used_natural[sc]];
-                const float yr = static_cast<float>(yv.x), yi = static_cast<float>(yv.y);
-                const float2 p = a.pilot_grid[step * A + sc];
-                float den = p.x * p.x + p.y * p.y;
-                if (den < 1e-9f) den = 1e-9f;
-                const float tr = yr * p.x - yi * (-p.y);
-                const float ti = yr * (-p.y) + yi * p.x;
-                hp[step * A + sc] = make_float2(tr / den, ti / den);
-            }
-        } else {
-            const int di = step - a.n_pilot;
-            const int p0 = a.interp_p0[di], p1 = a.interp_p1[di];
-            const float w0 = a.interp_w0[di], w1 = a.interp_w1[di];
-            for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
-                const int obit = (di * A + sc) * B;
-                if (obit >= a.llr_per_frame) continue;  // pad resource elements
-                float hr = 1.0f, hi = 0.0f;
-                if (a.use_pilots) {
-                    const float2 h0 = hp[p0 * A + sc], h1 = hp[p1 * A + sc];
-                    hr = h0.x * w0 + h1.x * w1;
-                    hi = h0.y * w0 + h1.y * w1;
-                }
-                float h2 = hr * hr + hi * hi;
-                if (h2 < 1e-9f) h2 = 1e-9f;
-                const Cd yv = buf[a.used_natural[sc]];
-                const float yr = static_cast<float>(yv.x), yi = static_cast<float>(yv.y);
-                const float tr = yr * hr - yi * (-hi);
-                const float ti = yr * (-hi) + yi * hr;
-                const float den = h2 + nv;
-                const float er = tr / den, ei = ti / den;
-                double s2 = static_cast<double>(a.noise_var_d) / static_cast<double>(h2);
-                if (s2 < 1e-8) s2 = 1e-8;
-                float d[M];
+        float h2 = hr * hr + hi * hi;
+        if (h2 < 1e-9f) h2 = 1e-9f;
+        const Cd yv = buf[a.used_natural[sc]];
+        const float yr = static_cast<float>(yv.x), yi = static_cast<float>(yv.y);
+        const float tr = yr * hr - yi * (-hi);
+        const float ti = yr * (-hi) + yi * hr;
+        const float den = h2 + nv;
+        const float er = tr / den, ei = ti / den;
+        double s2 = static_cast<double>(a.noise_var_d) / static_cast<double>(h2);
+        if (s2 < 1e-8) s2 = 1e-8;
+        float d[M];
 #pragma unroll
-                for (int i = 0; i < M; ++i) {
-                    const float2 cpt = a.constellation[i];
-                    const float dx = er - cpt.x, dy = ei - cpt.y;
-                    d[i] = dx * dx + dy * dy;
-                }
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    float d0 = INFINITY, d1 = INFINITY;
-#pragma unroll
-                    for (int i = 0; i < M; ++i) {
-                        if (((i >> (B - 1 - b)) & 1) == 0)
-                            d0 = d[i] < d0 ? d[i] : d0;
-                        else
-                            d1 = d[i] < d1 ? d[i] : d1;
-                    }
-                    if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) / s2);
-                }
-            }
+        for (int i = 0; i < M; ++i) {
+            const float2 cpt = a.constellation[i];
+            const float dx = er - cpt.x, dy = ei - cpt.y;
+            d[i] = dx * dx + dy * dy;
         }
-        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            float d0 = INFINITY, d1 = INFINITY;
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                if (((i >> (B - 1 - b)) & 1) == 0)
+                    d0 = d[i] < d0 ? d[i] : d0;
+                else
+                    d1 = d[i] < d1 ? d[i] : d1;
+            }
+            if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) / s2);
+        }
     }
 }
 
@@ -320,17 +347,21 @@ void launch_ofdm_demod(const OfdmDemodArgs& a, int frames, hipStream_t s)
     while (n > 1 && (n & 3) == 0) n >>= 2;
     if (n != 1 || a.nfft < 16 || a.nfft > 1024) throw std::invalid_argument("demod FFT needs nfft = 4^k in [16, 1024]");
     if (a.n_pilot < 1 && a.use_pilots) throw std::invalid_argument("pilot channel estimation needs a pilot symbol");
-    const size_t lds = sizeof(double2) * a.nfft + sizeof(float2) * static_cast<size_t>(a.n_pilot) * a.active_sc;
-    if (lds > 64 * 1024) throw std::invalid_argument("demod LDS footprint above 64 KiB");
+    if (a.use_pilots && !a.h_est) throw std::invalid_argument("demod: missing channel-estimate scratch");
+    if (a.n_data < 1) throw std::invalid_argument("demod: no data symbol");
+    const size_t lds = sizeof(double2) * a.nfft;
+    if (a.use_pilots)
+        hipLaunchKernelGGL(ofdm_pilot_kernel, dim3(frames * a.n_pilot), dim3(kThreads), lds, s, a);
+    const dim3 grid(frames * a.n_data);
     switch (a.bits_per_symbol) {
-        case 1: hipLaunchKernelGGL(ofdm_demod_kernel<1>, dim3(frames), dim3(kThreads), lds, s, a); break;
-        case 2: hipLaunchKernelGGL(ofdm_demod_kernel<2>, dim3(frames), dim3(kThreads), lds, s, a); break;
-        case 4: hipLaunchKernelGGL(ofdm_demod_kernel<4>, dim3(frames), dim3(kThreads), lds, s, a); break;
-        case 6: hipLaunchKernelGGL(ofdm_demod_kernel<6>, dim3(frames), dim3(kThreads), lds, s, a); break;
+        case 1: hipLaunchKernelGGL(ofdm_data_kernel<1>, grid, dim3(kThreads), lds, s, a); break;
+        case 2: hipLaunchKernelGGL(ofdm_data_kernel<2>, grid, dim3(kThreads), lds, s, a); break;
+        case 4: hipLaunchKernelGGL(ofdm_data_kernel<4>, grid, dim3(kThreads), lds, s, a); break;
+        case 6: hipLaunchKernelGGL(ofdm_data_kernel<6>, grid, dim3(kThreads), lds, s, a); break;
         default: throw std::invalid_argument("bits_per_symbol must be 1, 2, 4 or 6");
     }
     const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) throw std::runtime_error(std::string("ofdm_demod_kernel: ") + hipGetErrorString(e));
+    if (e != hipSuccess) throw std::runtime_error(std::string("ofdm demod kernels: ") + hipGetErrorString(e));
 }
 
 }  // namespace sicc

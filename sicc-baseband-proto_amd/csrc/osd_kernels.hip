@@ -15,7 +15,7 @@
 //      position order; strict-less keeps the first best TEP.
 //
 // Mapping: one wavefront per frame.  The generator is held column-major: lane l
-// owns columns l, l+64, ... as 64-bit words over the k <= 64 rows, so the
+// owns columns l, l+64, ... as one or two 64-bit words over the k <= 128 rows, so the
 // pivot search is one ballot per column slot and a column swap is a lane
 // exchange.  After elimination the parity part is transposed into LDS rows
 // R[i] (ballots), and each lane evaluates whole TEPs: the disagreement
@@ -74,23 +74,45 @@ __device__ __forceinline__ int unrank_tep(long long t, int pool, int order, int*
 constexpr int kMaxOrder = 3;
 constexpr int kOsdWavesPerBlock = 4;
 
-template <int CPL>
+template <int CPL, int KW>
 struct OsdLds {
     float llr[64 * CPL];
     double rel[64 * CPL];
-    unsigned long long rows[64 * CPL];  // R[i][s], i < k
+    unsigned long long rows[64 * KW * CPL];  // R[i][s], i < k
     int rperm[64 * CPL];
-    int pool[64];
+    int pool[64 * KW];
     unsigned char code[64 * CPL];
 };
 
-template <int CPL>
+// Bit r (< 64 KW) of a column held as KW words.
+template <int KW>
+__device__ __forceinline__ bool colbit(const unsigned long long* w, int r)
+{
+    return ((w[r >> 6] >> (r & 63)) & 1ull) != 0ull;
+}
+
+// Any one in rows >= t.
+template <int KW>
+__device__ __forceinline__ bool rows_from(const unsigned long long* w, int t)
+{
+    const int tw = t >> 6;
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < KW; ++q)
+        if (q > tw)
+            any |= w[q] != 0ull;
+        else if (q == tw)
+            any |= (w[q] >> (t & 63)) != 0ull;
+    return any;
+}
+
+template <int CPL, int KW>
 __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, OsdIo io)
 {
     extern __shared__ unsigned char lds_raw[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    OsdLds<CPL>& sh = reinterpret_cast<OsdLds<CPL>*>(lds_raw)[wv];
+    OsdLds<CPL, KW>& sh = reinterpret_cast<OsdLds<CPL, KW>*>(lds_raw)[wv];
     const int n = code.n;
     const int k = code.k;
     const int total = io.count ? *io.count : io.host_count;
@@ -126,21 +148,24 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
         }
         wave_lds_sync();
         // 2. column words of the permuted generator
-        unsigned long long col[CPL];
+        unsigned long long col[CPL][KW];
         int sp[CPL];
 #pragma unroll
         for (int s = 0; s < CPL; ++s) {
             const int c = lane + 64 * s;
-            col[s] = c < n ? code.gen_cols[sh.rperm[c]] : 0ull;
+#pragma unroll
+            for (int q = 0; q < KW; ++q) col[s][q] = c < n ? code.gen_cols[static_cast<size_t>(sh.rperm[c]) * KW + q] : 0ull;
             sp[s] = c;
         }
         int status = 0;
         for (int t = 0; t < k; ++t) {
+            const int tw = t >> 6;
+            const unsigned long long tbit = 1ull << (t & 63);
             int pc = -1;
 #pragma unroll
             for (int s = 0; s < CPL; ++s) {
                 const int c = lane + 64 * s;
-                const bool cand = c >= t && c < n && (col[s] >> t) != 0ull;
+                const bool cand = c >= t && c < n && rows_from<KW>(col[s], t);
                 const unsigned long long b = __ballot(cand);
                 if (pc < 0 && b != 0ull) pc = 64 * s + __ffsll(static_cast<long long>(b)) - 1;
             }
@@ -148,58 +173,82 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 status = 1;
                 break;
             }
-            // pivot column word (uniform)
-            unsigned long long wpc = 0;
+            // pivot column words (uniform)
+            unsigned long long wpc[KW];
+#pragma unroll
+            for (int q = 0; q < KW; ++q) wpc[q] = 0ull;
 #pragma unroll
             for (int s = 0; s < CPL; ++s)
-                if (s == (pc >> 6)) wpc = __shfl(col[s], pc & 63);
-            const int prow = t + __ffsll(static_cast<long long>(wpc >> t)) - 1;
+#pragma unroll
+                for (int q = 0; q < KW; ++q)
+                    if (s == (pc >> 6)) wpc[q] = __shfl(col[s][q], pc & 63);
+            // first row >= t with a one in the pivot column
+            int prow = -1;
+#pragma unroll
+            for (int q = 0; q < KW; ++q) {
+                if (prow >= 0 || q < tw) continue;
+                const unsigned long long m = q == tw ? (wpc[q] >> (t & 63)) << (t & 63) : wpc[q];
+                if (m != 0ull) prow = 64 * q + __ffsll(static_cast<long long>(m)) - 1;
+            }
             if (prow != t) {
-                const unsigned long long mt = 1ull << t, mp = 1ull << prow;
+                const int pw = prow >> 6;
+                const unsigned long long pbit = 1ull << (prow & 63);
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
-                    const bool b1 = (col[s] & mt) != 0ull, b2 = (col[s] & mp) != 0ull;
-                    if (b1 != b2) col[s] ^= (mt | mp);
+                    const bool b1 = (col[s][tw] & tbit) != 0ull, b2 = (col[s][pw] & pbit) != 0ull;
+                    if (b1 != b2) {
+                        col[s][tw] ^= tbit;
+                        col[s][pw] ^= pbit;
+                    }
                 }
             }
             if (pc != t) {
-                unsigned long long wt = 0, wp = 0;
+                unsigned long long wt[KW], wp[KW];
                 int st = 0, spc = 0;
 #pragma unroll
+                for (int q = 0; q < KW; ++q) wt[q] = wp[q] = 0ull;
+#pragma unroll
                 for (int s = 0; s < CPL; ++s) {
-                    const unsigned long long a = __shfl(col[s], t & 63);
-                    const unsigned long long b = __shfl(col[s], pc & 63);
+#pragma unroll
+                    for (int q = 0; q < KW; ++q) {
+                        const unsigned long long a = __shfl(col[s][q], t & 63);
+                        const unsigned long long b = __shfl(col[s][q], pc & 63);
+                        if (s == (t >> 6)) wt[q] = a;
+                        if (s == (pc >> 6)) wp[q] = b;
+                    }
                     const int sa = __shfl(sp[s], t & 63);
                     const int sb = __shfl(sp[s], pc & 63);
-                    if (s == (t >> 6)) {
-                        wt = a;
-                        st = sa;
-                    }
-                    if (s == (pc >> 6)) {
-                        wp = b;
-                        spc = sb;
-                    }
+                    if (s == (t >> 6)) st = sa;
+                    if (s == (pc >> 6)) spc = sb;
                 }
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
                     const int c = lane + 64 * s;
                     if (c == t) {
-                        col[s] = wp;
+#pragma unroll
+                        for (int q = 0; q < KW; ++q) col[s][q] = wp[q];
                         sp[s] = spc;
                     } else if (c == pc) {
-                        col[s] = wt;
+#pragma unroll
+                        for (int q = 0; q < KW; ++q) col[s][q] = wt[q];
                         sp[s] = st;
                     }
                 }
             }
-            unsigned long long piv = 0;
+            unsigned long long clr[KW];
+#pragma unroll
+            for (int q = 0; q < KW; ++q) clr[q] = 0ull;
 #pragma unroll
             for (int s = 0; s < CPL; ++s)
-                if (s == (t >> 6)) piv = __shfl(col[s], t & 63);
-            const unsigned long long clr = piv & ~(1ull << t);
+#pragma unroll
+                for (int q = 0; q < KW; ++q)
+                    if (s == (t >> 6)) clr[q] = __shfl(col[s][q], t & 63);
+            clr[tw] &= ~tbit;
 #pragma unroll
             for (int s = 0; s < CPL; ++s)
-                if ((col[s] >> t) & 1ull) col[s] ^= clr;
+                if (col[s][tw] & tbit)
+#pragma unroll
+                    for (int q = 0; q < KW; ++q) col[s][q] ^= clr[q];
         }
         if (status) {
             if (lane == 0 && io.status) io.status[frame] = 1;
@@ -218,22 +267,30 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 hs[s] = false;
             }
         }
-        // base message (k <= 64: all in slot 0)
-        const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-        const unsigned long long base = __ballot(hs[0]) & kmask;
+        // base message: the hard decisions of the k systematic positions (column
+        // slots 0 .. KW-1 hold positions 0 .. 64 KW - 1)
+        unsigned long long base[KW];
+#pragma unroll
+        for (int q = 0; q < KW; ++q) {
+            const int lo = 64 * q;
+            const unsigned long long m = k - lo >= 64 ? ~0ull : (k > lo ? ((1ull << (k - lo)) - 1ull) : 0ull);
+            base[q] = __ballot(hs[q]) & m;
+        }
         unsigned long long d0[CPL];
 #pragma unroll
         for (int s = 0; s < CPL; ++s) {
             const int c = lane + 64 * s;
             const bool par = c >= k && c < n;
-            const bool pbit = (__popcll(base & col[s]) & 1) != 0;
-            d0[s] = __ballot(par && (pbit != hs[s]));
+            int pb = 0;
+#pragma unroll
+            for (int q = 0; q < KW; ++q) pb ^= __popcll(base[q] & col[s][q]) & 1;
+            d0[s] = __ballot(par && ((pb != 0) != hs[s]));
         }
         for (int i = 0; i < k; ++i) {
 #pragma unroll
             for (int s = 0; s < CPL; ++s) {
                 const int c = lane + 64 * s;
-                const unsigned long long b = __ballot(c >= k && c < n && ((col[s] >> i) & 1ull));
+                const unsigned long long b = __ballot(c >= k && c < n && colbit<KW>(col[s], i));
                 if (lane == 0) sh.rows[i * CPL + s] = b;
             }
         }
@@ -301,13 +358,17 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
         {
             int ix[kMaxOrder];
             const int o = unrank_tep(best_t, pool, p.order, ix);
-            unsigned long long fm = 0;
+            unsigned long long fm[KW];
+#pragma unroll
+            for (int q = 0; q < KW; ++q) fm[q] = 0ull;
             unsigned long long d[CPL];
 #pragma unroll
             for (int s = 0; s < CPL; ++s) d[s] = d0[s];
             for (int i = 0; i < o; ++i) {
                 const int ps = sh.pool[ix[i]];
-                fm |= 1ull << ps;
+#pragma unroll
+                for (int q = 0; q < KW; ++q)
+                    if (q == (ps >> 6)) fm[q] |= 1ull << (ps & 63);
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) d[s] ^= sh.rows[ps * CPL + s];
             }
@@ -316,10 +377,15 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 const int c = lane + 64 * s;
                 if (c < n) {
                     unsigned char bit;
-                    if (c < k)
-                        bit = static_cast<unsigned char>(((base ^ fm) >> c) & 1ull);
-                    else
+                    if (c < k) {
+                        unsigned long long w = 0ull;
+#pragma unroll
+                        for (int q = 0; q < KW; ++q)
+                            if (q == s) w = base[q] ^ fm[q];
+                        bit = static_cast<unsigned char>((w >> lane) & 1ull);
+                    } else {
                         bit = static_cast<unsigned char>((hs[s] ? 1 : 0) ^ ((d[s] >> lane) & 1ull));
+                    }
                     sh.code[sh.rperm[sp[s]]] = bit;
                 }
             }
@@ -339,30 +405,35 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
     }
 }
 
-template <int CPL>
+template <int CPL, int KW>
 void launch_osd_variant(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames, hipStream_t s)
 {
     int waves = max_frames < 2048 ? max_frames : 2048;
     if (waves < 1) waves = 1;
     const int blocks = (waves + kOsdWavesPerBlock - 1) / kOsdWavesPerBlock;
-    const size_t lds = sizeof(OsdLds<CPL>) * kOsdWavesPerBlock;
-    hipLaunchKernelGGL((osd_kernel<CPL>), dim3(blocks), dim3(64 * kOsdWavesPerBlock), lds, s, code, p, io);
+    const size_t lds = sizeof(OsdLds<CPL, KW>) * kOsdWavesPerBlock;
+    hipLaunchKernelGGL((osd_kernel<CPL, KW>), dim3(blocks), dim3(64 * kOsdWavesPerBlock), lds, s, code, p, io);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("osd_kernel launch: ") + hipGetErrorString(e));
 }
 
 }  // namespace
 
-bool osd_shape_supported(int n, int k) { return n > 0 && k > 0 && k <= 64 && k < n && n <= 256; }
+bool osd_shape_supported(int n, int k) { return n > 0 && k > 0 && k <= 128 && k < n && n <= 256; }
 
 void launch_osd(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames, hipStream_t s)
 {
-    if (!osd_shape_supported(code.n, code.k)) throw std::invalid_argument("OSD supports k <= 64, n <= 256");
+    if (!osd_shape_supported(code.n, code.k)) throw std::invalid_argument("OSD supports k <= 128, n <= 256");
     if (p.order < 0 || p.order > kMaxOrder) throw std::invalid_argument("OSD order must be 0..3");
     if (p.max_flip < 1 || p.max_tests < 1) throw std::invalid_argument("OSD max_flip and max_tests must be >= 1");
+    if (code.kw != (code.k + 63) / 64) throw std::invalid_argument("OSD generator word count does not match k");
     if (max_frames <= 0) return;
-    if (code.n <= 128) return launch_osd_variant<2>(code, p, io, max_frames, s);
-    return launch_osd_variant<4>(code, p, io, max_frames, s);
+    if (code.k <= 64) {
+        if (code.n <= 128) return launch_osd_variant<2, 1>(code, p, io, max_frames, s);
+        return launch_osd_variant<4, 1>(code, p, io, max_frames, s);
+    }
+    if (code.n <= 128) return launch_osd_variant<2, 2>(code, p, io, max_frames, s);
+    return launch_osd_variant<4, 2>(code, p, io, max_frames, s);
 }
 
 }  // namespace sicc

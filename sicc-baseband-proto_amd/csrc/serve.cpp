@@ -404,10 +404,16 @@ private:
                     llr = by[m][0]->d_llr;
                 } else {
                     if (s.pack_llr[m].size() == 0) s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
+                    // one gather workgroup per chunk of about 4096 LLRs, so a few
+                    // large jobs still spread over the whole GPU
+                    const int ch = std::max(1, 4096 / c.n);
                     GatherEntry* t = s.table + tbl;
-                    for (size_t i = 0; i < by[m].size(); ++i) t[i] = {by[m][i]->d_llr, by[m][i]->count, by[m][i]->row};
-                    launch_gather_rows(t, static_cast<int>(by[m].size()), s.pack_llr[m].get(), c.n, s.stream);
-                    tbl += static_cast<int>(by[m].size());
+                    int ne = 0;
+                    for (Job* j : by[m])
+                        for (int r = 0; r < j->count; r += ch)
+                            t[ne++] = {j->d_llr + static_cast<size_t>(r) * c.n, std::min(ch, j->count - r), j->row + r};
+                    launch_gather_rows(t, ne, s.pack_llr[m].get(), c.n, s.stream);
+                    tbl += ne;
                     llr = s.pack_llr[m].get();
                 }
                 if (s.pack_out[m].size() == 0) s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));

@@ -59,6 +59,28 @@ def test_bposd_bit_exact(ccsds, order, flips):
     assert np.array_equal(g["info"], o["info"])
 
 
+@pytest.mark.parametrize("order,flips", [(1, 128), (2, 48)])
+def test_bposd_k128_bit_exact(order, flips):
+    # CCSDS (256,128): OSD over a 128-row generator (two column words per lane)
+    import os
+
+    print(f"[case] BP-OSD (256,128) order={order} flips={flips}", flush=True)
+    path = os.path.join(sa.DATA_DIR, "CCSDS_ldpc_n256_k128.alist")
+    n, m, rp, ci = oracle.alist_parse(path)
+    code = oracle.Code(n, m, rp, ci, front_info=True)
+    _, _, llr = _frames(code, 600, 1.5, 91 + order)
+    cfg = sa.stream_config(n=256, k=128, max_iter=20, alpha=0.75)
+    with sa.Decoder.stream(cfg, alist=path) as dec:
+        dec.set_osd(order, flips, 4096)
+        g = dec.decode_host(llr)
+    o = code.bposd(llr, max_iter=20, alpha=0.75, order=order, max_flip=flips, max_tests=4096, threads=8)
+    assert o["used_osd"].sum() > 50
+    assert np.array_equal(g["used_osd"], o["used_osd"])
+    assert np.array_equal(g["bp_ok"], o["bp_ok"])
+    assert np.array_equal(g["code"], o["code"])
+    assert np.array_equal(g["info"], o["info"])
+
+
 def test_bposd_damping_offset_bit_exact(ccsds):
     print("[case] BP-OSD damping=0.15 offset=0.15", flush=True)
     _, _, llr = _frames(ccsds, 1500, 2.0, 55)
