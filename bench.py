@@ -222,13 +222,15 @@ def calibrate_mix(a, sa, np, torch):
 SERVE_FRACS = (0.1, 0.25, 0.5, 0.75, 0.9, 1.0)
 
 
-def serve_sweep(sa, np, rng, decoders, members, counts, ptrs, ks):
+def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
     """Peak (all jobs at t = 0), then Poisson arrivals at fractions of the peak,
     then the plain one-job-per-launch server on the same jobs."""
     n_jobs = len(members)
     bits = float(sum(int(c) * ks[m] for m, c in zip(members, counts)))
-    out = {"jobs": n_jobs}
+    out = {"jobs": n_jobs, "batching": {"slots": 4, "min_launch_codewords": a.serve_min_cw,
+                                        "max_wait_us": a.serve_max_wait_us}}
     with sa.Server(decoders, budget_bytes=16 << 30, slots=4, max_launch_codewords=1 << 16) as srv:
+        srv.set_batching(a.serve_min_cw, a.serve_max_wait_us)
         lat, wall = srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         peak_mbps = bits / wall / 1e6
         out["peak_mbps"] = round(peak_mbps, 2)
@@ -280,7 +282,7 @@ def leg_served(a, sa, np, torch, dec128):
     out = {"ccsds128": {"job_codewords": "uniform 1..512", "code": "CCSDS (128,64) BP+OSD order-1",
                         "ebn0_db": a.ebn0}}
     torch.cuda.synchronize()
-    out["ccsds128"].update(serve_sweep(sa, np, rng, [dec128], [0] * n_jobs, counts, ptrs, [64]))
+    out["ccsds128"].update(serve_sweep(a, sa, np, rng, [dec128], [0] * n_jobs, counts, ptrs, [64]))
     # config-3 mix at FER ~1e-3: one flooding BP decoder per code, LLR pools in HBM
     shapes = sorted(MIX_EBN0_SERVE)
     decs, pools, ks = [], [], []
@@ -300,7 +302,7 @@ def leg_served(a, sa, np, torch, dec128):
     out["mix"] = {"job_codewords": "uniform 1..512", "codes": len(shapes),
                   "code": "QC-LDPC mix n 128..2048, rates 1/2, 11/16, 3/4, flooding BP 50 it",
                   "ebn0_db": {f"n{n}_mb{mb}": e for (n, mb), e in sorted(MIX_EBN0_SERVE.items())}}
-    out["mix"].update(serve_sweep(sa, np, rng, decs, [int(x) for x in mem], cnt, mptr, ks))
+    out["mix"].update(serve_sweep(a, sa, np, rng, decs, [int(x) for x in mem], cnt, mptr, ks))
     for d in decs:
         d.close()
     return out
@@ -584,6 +586,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--served-jobs", type=int, default=20000)
     ap.add_argument("--served-mix-jobs", type=int, default=5000)
+    ap.add_argument("--serve-min-cw", type=int, default=8192,
+                    help="server batching: resident codewords that start a launch while another is in flight")
+    ap.add_argument("--serve-max-wait-us", type=float, default=200.0,
+                    help="server batching: oldest resident job's wait that starts a launch anyway")
     ap.add_argument("--mixed-frames", type=int, default=65536)
     ap.add_argument("--mixed-reps", type=int, default=5)
     ap.add_argument("--ofdm-frames", type=int, default=4096)

@@ -51,6 +51,12 @@ int sicc_server_fetch(unsigned long long handle, unsigned long long job_id, unsi
  * jobs pending). */
 int sicc_server_drain(unsigned long long handle);
 
+/* Batching policy.  While another launch is in flight, a new launch waits until
+ * min_launch_codewords are resident or the oldest resident job has waited
+ * max_wait_us; an idle GPU starts work at once.  0, 0 (the default): launch
+ * whenever a slot is free. */
+int sicc_server_set_batching(unsigned long long handle, int min_launch_codewords, double max_wait_us);
+
 /* hold != 0: queue arrivals without admitting or launching them (launches in
  * flight complete); hold == 0: resume, so the queued burst is packed. */
 int sicc_server_hold(unsigned long long handle, int hold);

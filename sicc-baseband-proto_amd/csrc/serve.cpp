@@ -237,6 +237,17 @@ public:
         cv_.notify_all();
     }
 
+    void set_batching(int min_cw, double max_wait_us)
+    {
+        if (min_cw < 0 || max_wait_us < 0.0) throw std::invalid_argument("set_batching: negative limit");
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            min_cw_ = min_cw;
+            max_wait_us_ = max_wait_us;
+        }
+        cv_.notify_all();
+    }
+
     void drain()
     {
         std::unique_lock<std::mutex> lk(mu_);
@@ -368,11 +379,28 @@ private:
         return any;
     }
 
+    // Batching policy: with another launch in flight, a launch waits until
+    // min_cw_ codewords are resident or the oldest resident job has waited
+    // max_wait_us_; with the GPU idle it starts at once.
+    bool launch_ready() const
+    {
+        if (resident_.empty()) return false;
+        if (min_cw_ <= 0) return true;
+        if (std::none_of(slots_.begin(), slots_.end(), [](const Slot& s) { return s.busy; })) return true;
+        long long cw = 0;
+        for (const auto& j : resident_) {
+            cw += j->count;
+            if (cw >= min_cw_) return true;
+        }
+        return static_cast<double>(now_us() - resident_.front()->t_submit) >= max_wait_us_;
+    }
+
     bool launch()
     {
         bool any = false;
         for (auto& s : slots_) {
             if (s.busy || resident_.empty()) continue;
+            if (!launch_ready()) break;
             // take jobs in admission order within the launch limits
             int total = 0;
             while (!resident_.empty()) {
@@ -460,6 +488,8 @@ private:
     sicc_server_stats_t st_{};
     bool stop_ = false;
     bool hold_ = false;
+    int min_cw_ = 0;
+    double max_wait_us_ = 0.0;
     bool failed_ = false;
     std::string fail_msg_;
     std::thread thread_;
@@ -545,6 +575,14 @@ int sicc_server_drain(unsigned long long handle)
 {
     return guarded([&] {
         slookup(handle)->drain();
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_set_batching(unsigned long long handle, int min_launch_codewords, double max_wait_us)
+{
+    return guarded([&] {
+        slookup(handle)->set_batching(min_launch_codewords, max_wait_us);
         return CUDA_OSD_STATUS_OK;
     });
 }

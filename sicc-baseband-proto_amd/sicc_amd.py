@@ -412,6 +412,10 @@ class Server:
     def drain(self):
         check(self.lib.sicc_server_drain(self.handle))
 
+    def set_batching(self, min_launch_codewords: int, max_wait_us: float):
+        self.lib.sicc_server_set_batching.argtypes = [C.c_ulonglong, C.c_int, C.c_double]
+        check(self.lib.sicc_server_set_batching(self.handle, min_launch_codewords, max_wait_us))
+
     def hold(self, on: bool):
         self.lib.sicc_server_hold.argtypes = [C.c_ulonglong, C.c_int]
         check(self.lib.sicc_server_hold(self.handle, int(on)))

@@ -114,5 +114,13 @@ def test_native_driver_latencies():
                               arrivals)
         assert (lat > 0).all() and wall > 0
         assert srv.stats().jobs_completed == 80
+        # batching policy: launches wait for 4096 resident codewords or 300 us while one is in flight
+        srv.set_batching(4096, 300.0)
+        lat2, _ = srv.drive([m for m, _ in jobs], [l.shape[0] for _, l in jobs], [t.data_ptr() for t in dev],
+                            np.zeros(len(jobs)))
+        st = srv.stats()
+        assert st.jobs_completed == 160 and (lat2 > 0).all()
+        with pytest.raises(sa.SiccError):
+            srv.set_batching(-1, 0.0)
     for d in decs:
         d.close()
