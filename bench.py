@@ -39,6 +39,25 @@ for _p in (PKG, os.path.join(ROOT, "oracle")):
 
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # CUs x bytes/clk x GHz
+VALU_PEAK_GIPS = 256 * 2.4  # wave64 VALU instructions issued per s (one per CU per clock), in G
+
+# HBM bytes and VALU instructions per headline bp_flood_kernel dispatch, from the
+# rocprofv3 counter passes summarized in profiles/bp_flood_ccsds128_1M_pmc.md
+# (FETCH_SIZE x2 and WRITE_SIZE x1 after calibration on a copy of known size);
+# they hold for the profiled workload only.
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "bp_flood_ccsds128_1M_pmc.md")
+
+
+def profiled_counts(a, B):
+    if B != 1 << 20 or a.ebn0 != 4.0 or a.osd_order != 1 or not os.path.exists(PMC_SUMMARY):
+        return None
+    got = {}
+    with open(PMC_SUMMARY) as f:
+        for line in f:
+            for key in ("traffic_bytes_per_launch", "valu_insts_per_launch"):
+                if line.startswith(key + ":"):
+                    got[key] = int(line.split(":")[1])
+    return got if len(got) == 2 else None
 
 # Eb/N0 (dB) per (n, block rows) of the mixed-code workload where the frame error
 # rate of 50-iteration flooding min-sum (alpha 0.75) crosses 1e-2 (mixed leg) and
@@ -608,8 +627,8 @@ def main() -> int:
                     help="run one device copy of the LLR buffer (known bytes) to scale HBM counters")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this decoder runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not built", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this decoder is measured on one GPU; sharding across GPUs "
+                                   "is not built", "n_gpus": a.gpus}))
         return 2
     legs = {x for x in a.legs.split(",") if x}
     if a.no_cpu:
@@ -653,6 +672,13 @@ def main() -> int:
 
     bp_gbs = h["hbm_bytes"] / (h["bp_ms"] * 1e-3) / 1e9
     lds_gbs = h["lds_bytes"] / (h["bp_ms"] * 1e-3) / 1e9
+    pmc = profiled_counts(a, h["B"])
+    valu = None
+    if pmc:
+        gips = pmc["valu_insts_per_launch"] / (h["bp_ms"] * 1e-3) / 1e9
+        valu = {"achieved": round(gips, 1), "peak": VALU_PEAK_GIPS, "unit": "G wave-instructions/s",
+                "frac": round(gips / VALU_PEAK_GIPS, 4), "insts_per_launch": pmc["valu_insts_per_launch"],
+                "source": "profiles/bp_flood_ccsds128_1M_pmc.md"}
     out = {
         "metric": "decoded info Mbps at FER<=1e-3 (batched vs plain); p50/p99 latency",
         "value": round(h["value"], 3),
@@ -675,10 +701,12 @@ def main() -> int:
         "plain": h.get("plain"),
         "batched_vs_plain": round(h["value"] / h["plain"]["value"], 2) if h.get("plain") else None,
         "roofline": {"bound": "hbm", "achieved": round(bp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(bp_gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "bp_flood_kernel",
+                     "frac": round(bp_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": pmc["traffic_bytes_per_launch"] if pmc else None, "kernel": "bp_flood_kernel",
                      "kernel_ms": round(h["bp_ms"], 4), "bytes_per_launch": h["hbm_bytes"]},
         "lds_roofline": {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
                          "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "bytes_per_launch": h["lds_bytes"]},
+        "valu_roofline": valu,
         "osd_kernel_ms": round(h["osd_ms"], 4),
         "cpu_baseline": cpu,
         "cpu_reference_min_sum": cpu_ref,

@@ -29,6 +29,7 @@ struct OfdmSyncArgs {
     double pre_energy = 0.0;           // sum of |preamble|^2 (float norms, double sum)
     OfdmSyncResult* out = nullptr;     // [frames]
     int window_in_lds = 1;             // set by the launcher
+    int prefix = 0;                    // set by the launcher: prefix-sum coarse search
 };
 
 struct OfdmDemodArgs {

@@ -82,22 +82,92 @@ __global__ __launch_bounds__(kThreads) void ofdm_sync_kernel(OfdmSyncArgs a)
     // coarse: M(d) = |sum conj(x[d+i]) x[d+i+L]|^2 / max(E1 E2, 1e-18)
     double best_m = 0.0;
     int best_d = 0x7fffffff;
-    for (int d = threadIdx.x; d <= limit; d += blockDim.x) {
-        double pr = 0.0, pi = 0.0, e1 = 0.0, e2 = 0.0;
-        for (int i = 0; i < L; ++i) {
-            const float2 u = w[d + i];
-            const float2 v = w[d + i + L];
-            const double ur = u.x, ui = u.y, vr = v.x, vi = v.y;
-            pr += ur * vr + ui * vi;
-            pi += ur * vi - ui * vr;
-            e1 += static_cast<double>(u.x * u.x + u.y * u.y);
-            e2 += static_cast<double>(v.x * v.x + v.y * v.y);
+    if (a.prefix) {
+        // Windows staged in LDS: prefix sums S[j] = sum_{i<j} conj(x[i]) x[i+L] and
+        // Q[j] = sum_{i<j} |x[i]|^2 (float norms, double sums), then
+        // P(d) = S[d+L] - S[d], E1 = Q[d+L] - Q[d], E2 = Q[d+2L] - Q[d+L]: O(window)
+        // work per frame instead of O(candidates * L).
+        const int nc = wlen - L;
+        double* Sr = reinterpret_cast<double*>(win + wlen);
+        double* Si = Sr + nc + 1;
+        double* Q = Si + nc + 1;
+        const int t = threadIdx.x;
+        const int per_c = (nc + kThreads - 1) / kThreads;
+        const int per_q = (wlen + kThreads - 1) / kThreads;
+        const int lc = min(nc, t * per_c), hc = min(nc, lc + per_c);
+        const int lq = min(wlen, t * per_q), hq = min(wlen, lq + per_q);
+        double sr = 0.0, si = 0.0, sq = 0.0;
+        for (int i = lc; i < hc; ++i) {
+            const float2 u = w[i], v = w[i + L];
+            sr += static_cast<double>(u.x) * v.x + static_cast<double>(u.y) * v.y;
+            si += static_cast<double>(u.x) * v.y - static_cast<double>(u.y) * v.x;
         }
-        const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
-        const double m = (pr * pr + pi * pi) / den;
-        if (m > best_m) {
-            best_m = m;
-            best_d = d;
+        for (int i = lq; i < hq; ++i) sq += static_cast<double>(w[i].x * w[i].x + w[i].y * w[i].y);
+        bm[t] = sr;
+        __shared__ double s_i[kThreads], s_q[kThreads];
+        s_i[t] = si;
+        s_q[t] = sq;
+        __syncthreads();
+        if (t == 0) {  // exclusive scan of the per-thread totals
+            double ar = 0.0, ai = 0.0, aq = 0.0;
+            for (int u = 0; u < kThreads; ++u) {
+                const double xr = bm[u], xi = s_i[u], xq = s_q[u];
+                bm[u] = ar;
+                s_i[u] = ai;
+                s_q[u] = aq;
+                ar += xr;
+                ai += xi;
+                aq += xq;
+            }
+            Sr[0] = 0.0;
+            Si[0] = 0.0;
+            Q[0] = 0.0;
+        }
+        __syncthreads();
+        sr = bm[t];
+        si = s_i[t];
+        sq = s_q[t];
+        for (int i = lc; i < hc; ++i) {
+            const float2 u = w[i], v = w[i + L];
+            sr += static_cast<double>(u.x) * v.x + static_cast<double>(u.y) * v.y;
+            si += static_cast<double>(u.x) * v.y - static_cast<double>(u.y) * v.x;
+            Sr[i + 1] = sr;
+            Si[i + 1] = si;
+        }
+        for (int i = lq; i < hq; ++i) {
+            sq += static_cast<double>(w[i].x * w[i].x + w[i].y * w[i].y);
+            Q[i + 1] = sq;
+        }
+        __syncthreads();
+        for (int d = t; d <= limit; d += blockDim.x) {
+            const double pr = Sr[d + L] - Sr[d], pi = Si[d + L] - Si[d];
+            const double e1 = Q[d + L] - Q[d], e2 = Q[d + 2 * L] - Q[d + L];
+            const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
+            const double m = (pr * pr + pi * pi) / den;
+            if (m > best_m) {
+                best_m = m;
+                best_d = d;
+            }
+        }
+        __syncthreads();  // bm is reused by the argmax below
+    } else {
+        for (int d = threadIdx.x; d <= limit; d += blockDim.x) {
+            double pr = 0.0, pi = 0.0, e1 = 0.0, e2 = 0.0;
+            for (int i = 0; i < L; ++i) {
+                const float2 u = w[d + i];
+                const float2 v = w[d + i + L];
+                const double ur = u.x, ui = u.y, vr = v.x, vi = v.y;
+                pr += ur * vr + ui * vi;
+                pi += ur * vi - ui * vr;
+                e1 += static_cast<double>(u.x * u.x + u.y * u.y);
+                e2 += static_cast<double>(v.x * v.x + v.y * v.y);
+            }
+            const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
+            const double m = (pr * pr + pi * pi) / den;
+            if (m > best_m) {
+                best_m = m;
+                best_d = d;
+            }
         }
     }
     // argmax, first index on ties (strict > in index order)
@@ -333,8 +403,13 @@ void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s)
     const int wlen = std::min(last, limit + a.radius) + 2 * L;
     OfdmSyncArgs b = a;
     size_t lds = sizeof(float2) * static_cast<size_t>(wlen);
+    const size_t prefix_lds = lds + sizeof(double) * (2 * static_cast<size_t>(wlen - L + 1) + wlen + 1);
     b.window_in_lds = lds <= 56 * 1024 ? 1 : 0;
-    if (!b.window_in_lds) lds = 0;
+    b.prefix = prefix_lds <= 56 * 1024 ? 1 : 0;
+    if (b.prefix)
+        lds = prefix_lds;
+    else if (!b.window_in_lds)
+        lds = 0;
     hipLaunchKernelGGL(ofdm_sync_kernel, dim3(frames), dim3(kThreads), lds, s, b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("ofdm_sync_kernel: ") + hipGetErrorString(e));
