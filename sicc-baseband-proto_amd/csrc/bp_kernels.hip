@@ -107,7 +107,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
     for (int t = 0; t < NPL; ++t) {
         const int v = tid + T * t;
         cm[t] = v < n ? code.col_meta[v] : 0;
-        const int dg0 = __shfl(cm[t] & 255, 0);
+        const int dg0 = __builtin_amdgcn_readfirstlane(cm[t] & 255);
         if (__ballot(v >= n || (cm[t] & 255) != dg0) == 0ull) uni |= 1u << t;
     }
     if (tid == 0) {
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
         int b = 0;
         if (TEAM == 1) {
             if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
-            b = __shfl(b, 0);
+            b = __builtin_amdgcn_readlane(b, 0);
         } else {
             if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
             __syncthreads();
@@ -208,14 +208,16 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                 for (int j = 0; j < MAXDC; ++j) {
                     const float a = app[em[s][j] >> 16];
                     hp ^= __float_as_uint(a);  // canonical APP: sign bit == (APP < 0)
-                    // q differs from the exact APP's only in the sign of a zero, which
-                    // the canonicalization below removes
+                    // The LDS APP is never -0, so a - r_old is never -0 either (x - y is
+                    // -0 only for x = -0, y = +0): q's sign bit is (q < 0) as in the
+                    // reference, whose q can differ only in the sign of a zero.  Damped
+                    // q can underflow to -0 and is canonicalized.
                     float q = a - rold[s][j];
                     if (DAMP) {
                         if (it > 0) q = (1.0f - damp) * q + damp * qold[s][j];
                         qold[s][j] = q;
+                        q = q + 0.0f;
                     }
-                    q = q + 0.0f;
                     qv[s][j] = q;
                     sp ^= __float_as_uint(q);
                     const float av = fabsf(q);
@@ -256,18 +258,16 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
                 // The first minimum edge takes min2; any other edge equal to min1
                 // implies min2 == min1, so |q| == min1 selects the same magnitude.
                 const float m1 = m1s[s];
-                float a1 = m1, a2 = m2s[s];
-                if (off != 0.0f) {
-                    a1 = fmaxf(a1 - off, 0.0f);
-                    a2 = fmaxf(a2 - off, 0.0f);
-                }
-                a1 = alpha * a1;
-                a2 = alpha * a2;
+                // with off == 0, m - 0 == m and max(m, 0) == m for every m >= +0, so
+                // the offset needs no test
+                const float a1 = alpha * fmaxf(m1 - off, 0.0f);
+                const float a2 = alpha * fmaxf(m2s[s] - off, 0.0f);
                 // keep the two products per row (not one per edge after the select)
-                asm volatile("" : "+v"(a1), "+v"(a2));
+                float b1 = a1, b2 = a2;
+                asm volatile("" : "+v"(b1), "+v"(b2));
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
-                    const float mag = (fabsf(qv[s][j]) == m1) ? a2 : a1;
+                    const float mag = (fabsf(qv[s][j]) == m1) ? b2 : b1;
                     const float r =
                         __uint_as_float(__float_as_uint(mag) | ((sps[s] ^ __float_as_uint(qv[s][j])) & 0x80000000u));
                     rold[s][j] = r;
@@ -313,14 +313,14 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
 #pragma unroll
             for (int t = 0; t < NPL; ++t)
                 if (tid + T * t < n) mn = fminf(mn, fabsf(appr[t]));
-            for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
-            if (TEAM > 1) {
-                if (tid == 0) red[3] = __float_as_int(FLT_MAX);
-                __syncthreads();
-                if (lane == 0) atomicMin(red + 3, __float_as_int(mn));  // non-negative floats order as ints
-                __syncthreads();
-                mn = __int_as_float(red[3]);
-            }
+            // Through an LDS minimum (non-negative floats order as their bit patterns):
+            // a lane-shuffle tree would keep its lane addresses in registers for the
+            // whole kernel.
+            if (tid == 0) red[3] = __float_as_int(FLT_MAX);
+            team_sync<TEAM>();
+            atomicMin(red + 3, __float_as_int(mn));
+            team_sync<TEAM>();
+            mn = __int_as_float(red[3]);
             accept = mn >= p.min_abs_accept;
         }
         if (io.code_bits || io.app) {
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
             int slot = 0;
             if (TEAM == 1) {
                 if (lane == 0) slot = atomicAdd(io.osd_count, 1);
-                slot = __shfl(slot, 0);
+                slot = __builtin_amdgcn_readlane(slot, 0);
             } else {
                 if (tid == 0) red[2] = atomicAdd(io.osd_count, 1);
                 __syncthreads();
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
             int b = 0;
             if (TEAM == 1) {
                 if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
-                b = __shfl(b, 0);
+                b = __builtin_amdgcn_readlane(b, 0);
             } else {
                 if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
                 __syncthreads();
@@ -398,9 +398,9 @@ template <int MPL, int NPL, int MAXDC, int TEAM>
 void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, int tpb_req,
                     hipStream_t s)
 {
-    const int e_pad = (code.E + 1 + 63) & ~63;  // one spare slot for padded edges
     const bool damp = p.damping != 0.0f;
     constexpr int T = 64 * TEAM;
+    const int e_pad = (code.E + 1 + 63) & ~63;  // one spare slot for padded edges
     const size_t team_bytes = sizeof(float) * static_cast<size_t>(8 + T * NPL + 4 + e_pad);
     // Up to 4 single-wave teams per group; a multi-wave team synchronizes with
     // group barriers, so it must be alone in its group.  Within 64 KiB of LDS.
