@@ -151,6 +151,14 @@ void OfdmRx::init(const float2* preamble, const std::vector<int>& nat, const std
         tw[m] = make_double2(std::cos(a), std::sin(a));
     }
     const auto con = qam_constellation(cfg.bits_per_symbol);
+    {
+        // The demapper evaluates distances per axis: point (gi << qb) | gq must sit
+        // at (x of point gi << qb, y of point gq).
+        const int B = cfg.bits_per_symbol, ax = B == 1 ? 1 : B / 2, qb = B - ax;
+        for (int idx = 0; idx < static_cast<int>(con.size()); ++idx)
+            if (con[idx].x != con[(idx >> qb) << qb].x || con[idx].y != con[idx & ((1 << qb) - 1)].y)
+                throw std::logic_error("constellation is not a product of per-axis levels");
+    }
 
     auto up = [](auto& buf, const auto* src, size_t count) {
         buf.alloc(std::max<size_t>(count, 1));

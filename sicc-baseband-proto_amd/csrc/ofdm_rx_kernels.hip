@@ -282,15 +282,25 @@ __device__ void symbol_fft(const OfdmDemodArgs& a, const float2* x, double cfo, 
 {
     const int N = a.nfft;
     const int base = a.preamble_len + sym * (N + a.cp) + a.cp;
+    // Derotation factor e^{-2 pi i cfo n / fs}, rounded to float as the reference
+    // applies it.  Each thread evaluates it in double at its first sample and steps
+    // by the thread stride with double complex products (N / blockDim - 1 of them):
+    // a relative error near 1e-15, far below the float rounding of the factor.
+    const double hc = -2.0 * cfo / a.fs;  // phase per sample in units of pi
+    double s0, c0, sd, cd;
+    sincospi(hc * static_cast<double>(base + static_cast<int>(threadIdx.x)), &s0, &c0);
+    sincospi(hc * static_cast<double>(blockDim.x), &sd, &cd);
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
         const int n = base + k;
-        const double ang = -2.0 * kPiD * cfo * static_cast<double>(n) / a.fs;
-        const float c = static_cast<float>(cos(ang));
-        const float s = static_cast<float>(sin(ang));
+        const float c = static_cast<float>(c0);
+        const float s = static_cast<float>(s0);
         const float2 v = x[n];
         const float re = v.x * c - v.y * s;
         const float im = v.x * s + v.y * c;
         buf[k] = {static_cast<double>(re), static_cast<double>(im)};
+        const double cn = c0 * cd - s0 * sd;
+        s0 = s0 * cd + c0 * sd;
+        c0 = cn;
     }
     __syncthreads();
     fft_radix4(buf, a.twiddle, N);
@@ -344,7 +354,9 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
     if (di * A * B >= a.llr_per_frame) return;  // a symbol of pad resource elements only
     const float2* x = a.rx + static_cast<size_t>(f) * a.span + sr.start;
     symbol_fft(a, x, sr.cfo_hz_d, a.data_symbols[di], buf);
-    constexpr int M = 1 << B;
+    constexpr int AX = B == 1 ? 1 : B / 2;  // leading (in-phase) bits of a symbol index
+    constexpr int QB = B - AX;               // trailing (quadrature) bits
+    constexpr int LI = 1 << AX, LQ = 1 << QB;
     const float nv = a.noise_var;
     const int p0 = a.interp_p0[di], p1 = a.interp_p1[di];
     const float w0 = a.interp_w0[di], w1 = a.interp_w1[di];
@@ -369,24 +381,58 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
         const float er = tr / den, ei = ti / den;
         double s2 = static_cast<double>(a.noise_var_d) / static_cast<double>(h2);
         if (s2 < 1e-8) s2 = 1e-8;
-        float d[M];
+        // one division per resource element: x * (1 / s2) is within 2 double ulps
+        // of x / s2, so the float LLR differs only where x / s2 falls that close
+        // to a float rounding boundary
+        const double inv_s2 = 1.0 / s2;
+        // The Gray QAM table is a product of an in-phase and a quadrature level set:
+        // point (gi << QB) | gq sits at (X[gi], Y[gq]), so its distance is
+        // fl(DX[gi] + DY[gq]).  fl(a + b) is monotone in each operand, hence the
+        // minimum over the points whose bit b is 0 (or 1) equals fl(min DX + min DY)
+        // over the matching level subsets: the same floats as the 2^B-point scan.
+        float dxs[LI], dys[LQ];
 #pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const float2 cpt = a.constellation[i];
-            const float dx = er - cpt.x, dy = ei - cpt.y;
-            d[i] = dx * dx + dy * dy;
+        for (int g = 0; g < LI; ++g) {
+            const float dx = er - a.constellation[g << QB].x;
+            dxs[g] = dx * dx;
         }
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            float d0 = INFINITY, d1 = INFINITY;
+        for (int g = 0; g < LQ; ++g) {
+            const float dy = ei - a.constellation[g].y;
+            dys[g] = dy * dy;
+        }
+        float mx = INFINITY, my = INFINITY;
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                if (((i >> (B - 1 - b)) & 1) == 0)
-                    d0 = d[i] < d0 ? d[i] : d0;
-                else
-                    d1 = d[i] < d1 ? d[i] : d1;
+        for (int g = 0; g < LI; ++g) mx = dxs[g] < mx ? dxs[g] : mx;
+#pragma unroll
+        for (int g = 0; g < LQ; ++g) my = dys[g] < my ? dys[g] : my;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            float d0, d1;
+            if (b < AX) {
+                float m0 = INFINITY, m1 = INFINITY;
+#pragma unroll
+                for (int g = 0; g < LI; ++g) {
+                    if (((g >> (AX - 1 - b)) & 1) == 0)
+                        m0 = dxs[g] < m0 ? dxs[g] : m0;
+                    else
+                        m1 = dxs[g] < m1 ? dxs[g] : m1;
+                }
+                d0 = m0 + my;
+                d1 = m1 + my;
+            } else {
+                float m0 = INFINITY, m1 = INFINITY;
+#pragma unroll
+                for (int g = 0; g < LQ; ++g) {
+                    if (((g >> (B - 1 - b)) & 1) == 0)
+                        m0 = dys[g] < m0 ? dys[g] : m0;
+                    else
+                        m1 = dys[g] < m1 ? dys[g] : m1;
+                }
+                d0 = mx + m0;
+                d1 = mx + m1;
             }
-            if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) / s2);
+            if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) * inv_s2);
         }
     }
 }
