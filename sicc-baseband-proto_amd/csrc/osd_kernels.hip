@@ -57,6 +57,25 @@ __device__ __forceinline__ int unrank_tep(long long t, int pool, int order, int*
         if (t < c) break;
         t -= c;
     }
+    if (o == 1) {
+        idx[0] = static_cast<int>(t);
+        return 1;
+    }
+    if (o == 2) {
+        // pairs (a, b), a < b, in lexicographic order: S(a) = a (2 pool - a - 1) / 2
+        // pairs precede first index a; take the largest a with S(a) <= t
+        const long long P = pool;
+        auto S = [P](long long x) { return x * (2 * P - x - 1) / 2; };
+        const double bq = static_cast<double>(2 * P - 1);
+        long long a = static_cast<long long>((bq - sqrt(bq * bq - 8.0 * static_cast<double>(t))) * 0.5);
+        if (a < 0) a = 0;
+        if (a > P - 2) a = P - 2;
+        while (a > 0 && S(a) > t) --a;
+        while (a + 1 <= P - 2 && S(a + 1) <= t) ++a;
+        idx[0] = static_cast<int>(a);
+        idx[1] = static_cast<int>(a + 1 + (t - S(a)));
+        return 2;
+    }
     int prev = -1;
     for (int pos = 0; pos < o; ++pos) {
         int a = prev + 1;
@@ -69,6 +88,15 @@ __device__ __forceinline__ int unrank_tep(long long t, int pool, int order, int*
         prev = a;
     }
     return o;
+}
+
+// Value of x in lane l, for a wave-uniform l: v_readlane into scalar registers,
+// no LDS round trip (the elimination broadcasts a pivot column at every step).
+__device__ __forceinline__ unsigned long long lane_u64(unsigned long long x, int l)
+{
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(x), l);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(x >> 32), l);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
 constexpr int kMaxOrder = 3;
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
             for (int s = 0; s < CPL; ++s)
 #pragma unroll
                 for (int q = 0; q < KW; ++q)
-                    if (s == (pc >> 6)) wpc[q] = __shfl(col[s][q], pc & 63);
+                    if (s == (pc >> 6)) wpc[q] = lane_u64(col[s][q], pc & 63);
             // first row >= t with a one in the pivot column
             int prow = -1;
 #pragma unroll
@@ -209,17 +237,16 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 for (int q = 0; q < KW; ++q) wt[q] = wp[q] = 0ull;
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
+                    if (s == (t >> 6)) {
 #pragma unroll
-                    for (int q = 0; q < KW; ++q) {
-                        const unsigned long long a = __shfl(col[s][q], t & 63);
-                        const unsigned long long b = __shfl(col[s][q], pc & 63);
-                        if (s == (t >> 6)) wt[q] = a;
-                        if (s == (pc >> 6)) wp[q] = b;
+                        for (int q = 0; q < KW; ++q) wt[q] = lane_u64(col[s][q], t & 63);
+                        st = __builtin_amdgcn_readlane(sp[s], t & 63);
                     }
-                    const int sa = __shfl(sp[s], t & 63);
-                    const int sb = __shfl(sp[s], pc & 63);
-                    if (s == (t >> 6)) st = sa;
-                    if (s == (pc >> 6)) spc = sb;
+                    if (s == (pc >> 6)) {
+#pragma unroll
+                        for (int q = 0; q < KW; ++q) wp[q] = lane_u64(col[s][q], pc & 63);
+                        spc = __builtin_amdgcn_readlane(sp[s], pc & 63);
+                    }
                 }
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
@@ -242,7 +269,7 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
             for (int s = 0; s < CPL; ++s)
 #pragma unroll
                 for (int q = 0; q < KW; ++q)
-                    if (s == (t >> 6)) clr[q] = __shfl(col[s][q], t & 63);
+                    if (s == (t >> 6)) clr[q] = lane_u64(col[s][q], t & 63);
             clr[tw] &= ~tbit;
 #pragma unroll
             for (int s = 0; s < CPL; ++s)
@@ -408,10 +435,19 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
 template <int CPL, int KW>
 void launch_osd_variant(const OsdCodeDev& code, const OsdParams& p, const OsdIo& io, int max_frames, hipStream_t s)
 {
-    int waves = max_frames < 2048 ? max_frames : 2048;
-    if (waves < 1) waves = 1;
-    const int blocks = (waves + kOsdWavesPerBlock - 1) / kOsdWavesPerBlock;
     const size_t lds = sizeof(OsdLds<CPL, KW>) * kOsdWavesPerBlock;
+    // One wave per frame up to what the GPU holds at once (the frame count may
+    // live on the device; surplus waves find no frame and exit).
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(osd_kernel<CPL, KW>),
+                                                     64 * kOsdWavesPerBlock, lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        throw std::runtime_error("osd_kernel: occupancy query failed");
+    const long resident = static_cast<long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1) * kOsdWavesPerBlock;
+    long waves = max_frames < resident ? max_frames : resident;
+    if (waves < 1) waves = 1;
+    const int blocks = static_cast<int>((waves + kOsdWavesPerBlock - 1) / kOsdWavesPerBlock);
     hipLaunchKernelGGL((osd_kernel<CPL, KW>), dim3(blocks), dim3(64 * kOsdWavesPerBlock), lds, s, code, p, io);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("osd_kernel launch: ") + hipGetErrorString(e));
