@@ -23,6 +23,7 @@
 #include "fft_lds.hpp"
 #include "ofdm.hpp"
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -56,28 +57,53 @@ __device__ __forceinline__ float normal01(unsigned int seed, unsigned int idx)
     return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * kPiF * u2);
 }
 
-__global__ __launch_bounds__(kThreads) void tx_encode_generator_kernel(OfdmTxArgs a)
+// Codewords encoded per workgroup by the generator kernel: each generator
+// column word is read once from L2 for all of them.
+constexpr int kTxCwPerGroup = 8;
+
+__global__ __launch_bounds__(kThreads) void tx_encode_generator_kernel(OfdmTxArgs a, int total_cw, int cwg)
 {
-    extern __shared__ unsigned long long words[];  // [gen_words]
-    const int cw = blockIdx.x;                     // frame * blocks + block
-    const int f = cw / a.blocks;
-    const int b = cw - f * a.blocks;
-    const uint8_t* info = a.info + static_cast<size_t>(cw) * a.code_k;
-    for (int w = threadIdx.x; w < a.gen_words; w += blockDim.x) {
-        unsigned long long v = 0;
-        const int lim = min(64, a.code_k - w * 64);
-        for (int i = 0; i < lim; ++i) v |= static_cast<unsigned long long>(info[w * 64 + i] & 1u) << i;
-        words[w] = v;
+    extern __shared__ unsigned long long words[];  // [cwg][gen_words]
+    const int cw0 = blockIdx.x * cwg;
+    const int gw = a.gen_words;
+    const int lane = threadIdx.x & 63;
+    // information bits packed 64 to a word by wavefront ballots
+    for (int idx = threadIdx.x >> 6; idx < cwg * gw; idx += blockDim.x >> 6) {
+        const int j = idx / gw, w = idx - j * gw;
+        const int cw = cw0 + j;
+        const int i = w * 64 + lane;
+        const bool bit = cw < total_cw && i < a.code_k &&
+                         (a.info[static_cast<size_t>(cw) * a.code_k + i] & 1u) != 0u;
+        const unsigned long long v = __ballot(bit);
+        if (lane == 0) words[idx] = v;
     }
     __syncthreads();
-    uint8_t* out = a.coded + static_cast<size_t>(f) * a.coded_bits + static_cast<size_t>(b) * a.code_n;
+    const int nj = min(cwg, total_cw - cw0);
     for (int c = threadIdx.x; c < a.code_n; c += blockDim.x) {
-        const unsigned long long* g = a.gen_cols + static_cast<size_t>(c) * a.gen_words;
-        int par = 0;
-        for (int w = 0; w < a.gen_words; ++w) par ^= __popcll(words[w] & g[w]);
-        out[c] = static_cast<uint8_t>(par & 1);
+        const unsigned long long* g = a.gen_cols + static_cast<size_t>(c) * gw;
+        int par[kTxCwPerGroup];
+#pragma unroll
+        for (int j = 0; j < kTxCwPerGroup; ++j) par[j] = 0;
+        for (int w = 0; w < gw; ++w) {
+            const unsigned long long gv = g[w];
+#pragma unroll
+            for (int j = 0; j < kTxCwPerGroup; ++j)
+                if (j < nj) par[j] ^= __popcll(words[j * gw + w] & gv);
+        }
+#pragma unroll
+        for (int j = 0; j < kTxCwPerGroup; ++j) {
+            if (j >= nj) break;
+            const int cw = cw0 + j;
+            const int f = cw / a.blocks, b = cw - f * a.blocks;
+            a.coded[static_cast<size_t>(f) * a.coded_bits + static_cast<size_t>(b) * a.code_n + c] =
+                static_cast<uint8_t>(par[j] & 1);
+        }
     }
-    if (b == 0) {
+    // pad bits after the last codeword of each frame
+    for (int j = 0; j < nj; ++j) {
+        const int cw = cw0 + j;
+        const int f = cw / a.blocks, b = cw - f * a.blocks;
+        if (b != 0) continue;
         uint8_t* fr = a.coded + static_cast<size_t>(f) * a.coded_bits;
         for (int i = a.blocks * a.code_n + threadIdx.x; i < a.coded_bits; i += blockDim.x) fr[i] = 0;
     }
@@ -207,8 +233,10 @@ void launch_ofdm_tx(const OfdmTxArgs& a, int frames, hipStream_t s)
     } else {
         if (!a.gen_cols || a.gen_words != (a.code_k + 63) / 64 || a.gen_words > 8192)
             throw std::invalid_argument("TX generator shape");
-        hipLaunchKernelGGL(tx_encode_generator_kernel, dim3(frames * a.blocks), dim3(kThreads),
-                           sizeof(unsigned long long) * a.gen_words, s, a);
+        const int total = frames * a.blocks;
+        const int cwg = std::max(1, std::min(kTxCwPerGroup, 8192 / a.gen_words));  // <= 64 KiB of words
+        hipLaunchKernelGGL(tx_encode_generator_kernel, dim3((total + cwg - 1) / cwg), dim3(kThreads),
+                           sizeof(unsigned long long) * a.gen_words * cwg, s, a, total, cwg);
     }
     hipLaunchKernelGGL(tx_symbol_kernel, dim3(frames * a.num_symbols), dim3(kThreads), sizeof(double2) * a.nfft, s,
                        a);
