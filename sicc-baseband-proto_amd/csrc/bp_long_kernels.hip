@@ -71,21 +71,31 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
                     int cv[MAXW];
                     float m1 = FLT_MAX, m2 = FLT_MAX;
                     int mi = -1, neg = 0, deg = 0;
+                    // every slot of the group's width exists (padding included): the
+                    // column and message loads are issued together, not chained
 #pragma unroll
                     for (int j = 0; j < MAXW; ++j) {
                         if (j < width) {
                             const int slot = base + j * rows + pos;
-                            const int col = c.edge_col[slot];
+                            cv[j] = c.edge_col[slot];
+                            rv[j] = r[slot];
+                        } else {
+                            cv[j] = -1;
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < MAXW; ++j) {
+                        if (j < width) {
+                            const int slot = base + j * rows + pos;
+                            const int col = cv[j];
                             if (col >= 0) {
-                                const float rold = r[slot];
+                                const float rold = rv[j];
                                 float q = app[col] - rold;
                                 if (!LAYERED && damping != 0.0f) {
                                     if (it > 0) q = (1.0f - damping) * q + damping * qst[slot];
                                     qst[slot] = q;
                                 }
                                 qv[j] = q;
-                                rv[j] = rold;
-                                cv[j] = col;
                                 deg = j + 1;
                                 neg ^= q < 0.0f ? 1 : 0;
                                 const float av = fabsf(q);
@@ -124,18 +134,26 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
                 }
                 __syncthreads();
             }
-            // syndrome of the hard decision APP < 0
+            // syndrome of the hard decision APP < 0; a row's column indices are
+            // loaded four at a time before its APP reads, and once a violated row has
+            // been flagged the remaining groups are skipped (a stale read of the flag
+            // only costs time)
             const int fl = it & 1;
             for (int g = 0; g < c.groups; ++g) {
                 const int rows = c.grp_row[g + 1] - c.grp_row[g];
                 const int base = c.grp_edge[g];
                 const int width = c.grp_width[g];
-                for (int pos = tid; pos < rows; pos += kLongThreads) {
+                const int lim = *static_cast<volatile int*>(&s_flag[fl]) ? 0 : rows;
+                for (int pos = tid; pos < lim; pos += kLongThreads) {
+                    const int* ec = c.edge_col + base + pos;
                     int par = 0;
-                    for (int j = 0; j < width; ++j) {
-                        const int col = c.edge_col[base + j * rows + pos];
-                        if (col < 0) break;
-                        par ^= app[col] < 0.0f ? 1 : 0;
+                    for (int j0 = 0; j0 < width; j0 += 4) {
+                        int cols[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) cols[u] = j0 + u < width ? ec[(j0 + u) * rows] : -1;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (cols[u] >= 0) par ^= app[cols[u]] < 0.0f ? 1 : 0;
                     }
                     if (par) s_flag[fl] = 1;
                 }
