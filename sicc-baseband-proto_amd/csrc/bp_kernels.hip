@@ -71,8 +71,7 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* tab
 }
 
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
-__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
-                                                                              int e_pad)
+__device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
     extern __shared__ float lds[];
     constexpr int T = 64 * TEAM;     // threads per team
@@ -394,6 +393,24 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
     }
 }
 
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
+                                                                              int e_pad)
+{
+    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(code, p, io, e_pad);
+}
+
+// The single-wave shape of codes up to 128 variables and row degree 8 without
+// damping: its vector registers fit eight waves per SIMD, but the kernel
+// arguments held in scalar registers cap residency at seven.  Capping the
+// scalar budget spills a few of them to vector lanes and admits the eighth wave.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bp_flood_kernel_w8(BpCodeDev code,
+                                                                                                  BpParams p,
+                                                                                                  BpIo io, int e_pad)
+{
+    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad);
+}
+
 template <int MPL, int NPL, int MAXDC, int TEAM>
 void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, int tpb_req,
                     hipStream_t s)
@@ -409,6 +426,8 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
     const size_t lds = team_bytes * tpb;
     auto kern = damp ? bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>;
+    if constexpr (MPL == 1 && NPL == 2 && MAXDC == 8 && TEAM == 1)
+        if (!damp) kern = bp_flood_kernel_w8;
     // A persistent grid larger than what the GPU holds at once would leave its
     // last teams to start after the first ones finished all their work.
     int per_cu = 0, dev = 0, cus = 0;
