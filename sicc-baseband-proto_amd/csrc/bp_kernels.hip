@@ -430,13 +430,9 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
         if (!damp) kern = bp_flood_kernel_w8;
     // A persistent grid larger than what the GPU holds at once would leave its
     // last teams to start after the first ones finished all their work.
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), T * tpb, lds) !=
-            hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        throw std::runtime_error("bp_flood_kernel: occupancy query failed");
-    const long resident_teams = static_cast<long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1) * tpb;
+    const long resident_teams = static_cast<long>(resident_blocks_per_cu(reinterpret_cast<const void*>(kern), T * tpb,
+                                                                         lds)) *
+                                device_cu_count() * tpb;
     long teams = io.num_cw < max_waves / TEAM ? io.num_cw : max_waves / TEAM;
     if (teams > resident_teams) teams = resident_teams;
     if (teams < 1) teams = 1;

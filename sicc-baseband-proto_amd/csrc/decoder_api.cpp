@@ -32,7 +32,9 @@ namespace {
 
 HandleTable<Decoder>& table()
 {
-    static HandleTable<Decoder> t;
+    // never destroyed: handles still open at exit must not reach the HIP
+    // runtime from static destructors
+    static HandleTable<Decoder>& t = *new HandleTable<Decoder>;
     return t;
 }
 
@@ -289,7 +291,9 @@ struct MultiDec {
 
 HandleTable<MultiDec>& mtable()
 {
-    static HandleTable<MultiDec> t;
+    // never destroyed: handles still open at exit must not reach the HIP
+    // runtime from static destructors
+    static HandleTable<MultiDec>& t = *new HandleTable<MultiDec>;
     return t;
 }
 

@@ -471,7 +471,9 @@ struct PipelineHandle {
 
 sicc::HandleTable<PipelineHandle>& ptable()
 {
-    static sicc::HandleTable<PipelineHandle> t;
+    // never destroyed: handles still open at exit must not reach the HIP
+    // runtime from static destructors
+    static sicc::HandleTable<PipelineHandle>& t = *new sicc::HandleTable<PipelineHandle>;
     return t;
 }
 

@@ -98,6 +98,10 @@ void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int r
 // Kernel launchers.  They validate shapes on the host and throw
 // std::invalid_argument when a code does not fit an instantiated variant.
 void launch_zero_i32(int* p, int count, hipStream_t s);
+// Cached residency queries (per device; kernel, block size and dynamic LDS).
+int device_cu_count();
+int resident_blocks_per_cu(const void* kernel, int threads, size_t lds);
+
 bool bp_shape_supported(const BpCodeDev& c);
 // Launch shape of the BP kernel (the autotuner's search space):
 //   team            wavefronts cooperating on one codeword (1, 2, 4, 8, 16); 0 = default

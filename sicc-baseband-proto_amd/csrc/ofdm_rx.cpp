@@ -260,7 +260,9 @@ struct OfdmHandle {
 
 HandleTable<OfdmHandle>& otable()
 {
-    static HandleTable<OfdmHandle> t;
+    // never destroyed: handles still open at exit must not reach the HIP
+    // runtime from static destructors
+    static HandleTable<OfdmHandle>& t = *new HandleTable<OfdmHandle>;
     return t;
 }
 

@@ -438,13 +438,9 @@ void launch_osd_variant(const OsdCodeDev& code, const OsdParams& p, const OsdIo&
     const size_t lds = sizeof(OsdLds<CPL, KW>) * kOsdWavesPerBlock;
     // One wave per frame up to what the GPU holds at once (the frame count may
     // live on the device; surplus waves find no frame and exit).
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(osd_kernel<CPL, KW>),
-                                                     64 * kOsdWavesPerBlock, lds) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        throw std::runtime_error("osd_kernel: occupancy query failed");
-    const long resident = static_cast<long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1) * kOsdWavesPerBlock;
+    const long resident = static_cast<long>(resident_blocks_per_cu(reinterpret_cast<const void*>(osd_kernel<CPL, KW>),
+                                                                   64 * kOsdWavesPerBlock, lds)) *
+                          device_cu_count() * kOsdWavesPerBlock;
     long waves = max_frames < resident ? max_frames : resident;
     if (waves < 1) waves = 1;
     const int blocks = static_cast<int>((waves + kOsdWavesPerBlock - 1) / kOsdWavesPerBlock);

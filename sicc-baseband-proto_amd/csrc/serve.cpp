@@ -140,6 +140,7 @@ public:
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
         }
+        wake_.store(true, std::memory_order_release);
         cv_.notify_all();
         if (thread_.joinable()) thread_.join();
         for (auto& s : slots_) {
@@ -194,6 +195,7 @@ public:
             known_[id] = j;
             arrivals_.push_back(j);
             ++st_.jobs_submitted;
+            arrived_.fetch_add(1, std::memory_order_release);
         }
         cv_.notify_all();
     }
@@ -234,6 +236,7 @@ public:
             std::lock_guard<std::mutex> lk(mu_);
             hold_ = on;
         }
+        wake_.store(true, std::memory_order_release);
         cv_.notify_all();
     }
 
@@ -245,6 +248,7 @@ public:
             min_cw_ = min_cw;
             max_wait_us_ = max_wait_us;
         }
+        wake_.store(true, std::memory_order_release);
         cv_.notify_all();
     }
 
@@ -295,13 +299,23 @@ private:
                 bool progress = harvest();
                 if (!hold_) {
                     progress |= admit();
-                    progress |= launch();
+                    progress |= launch(lk);
                 }
                 const bool inflight = std::any_of(slots_.begin(), slots_.end(), [](const Slot& s) { return s.busy; });
                 if (!progress) {
                     if (inflight || (!hold_ && (!arrivals_.empty() || !resident_.empty()))) {
+                        // Wait for a launch to finish, a new job, or the oldest
+                        // resident job's batching deadline -- without the lock, so
+                        // submitters are not starved by this thread's polling.
+                        const uint64_t seen = arrived_.load(std::memory_order_acquire);
+                        const bool slot_free =
+                            std::any_of(slots_.begin(), slots_.end(), [](const Slot& s) { return !s.busy; });
+                        const uint64_t deadline = (resident_.empty() || !slot_free)
+                                                      ? UINT64_MAX
+                                                      : resident_.front()->t_submit +
+                                                            static_cast<uint64_t>(max_wait_us_);
                         lk.unlock();
-                        std::this_thread::yield();
+                        wait_event(seen, deadline);
                         lk.lock();
                     } else {
                         cv_.wait(lk, [&] { return stop_ || (!hold_ && (!arrivals_.empty() || !resident_.empty())); });
@@ -313,6 +327,19 @@ private:
             failed_ = true;
             fail_msg_ = e.what();
             cv_done_.notify_all();
+        }
+    }
+
+    // Lock not held: only this thread records and queries the slots' events.
+    void wait_event(uint64_t seen, uint64_t deadline)
+    {
+        for (;;) {
+            if (arrived_.load(std::memory_order_acquire) != seen) return;
+            for (auto& s : slots_)
+                if (s.busy && hipEventQuery(s.done) != hipErrorNotReady) return;
+            if (now_us() >= deadline) return;
+            if (wake_.exchange(false, std::memory_order_acq_rel)) return;  // stop, hold or policy change
+            std::this_thread::yield();
         }
     }
 
@@ -395,7 +422,11 @@ private:
         return static_cast<double>(now_us() - resident_.front()->t_submit) >= max_wait_us_;
     }
 
-    bool launch()
+    // Called with the lock held; the slot's jobs are taken under the lock, the
+    // packing and kernel launches run with it released (only this thread touches
+    // slots and the jobs' packing fields), so submit/poll/fetch callers do not
+    // wait behind the launch work.
+    bool launch(std::unique_lock<std::mutex>& lk)
     {
         bool any = false;
         for (auto& s : slots_) {
@@ -412,7 +443,30 @@ private:
                 resident_.pop_front();
             }
             if (s.jobs.empty()) continue;
-            if (admit_pending_) SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
+            const bool wait_admit = admit_pending_;
+            lk.unlock();
+            try {
+                issue(s, wait_admit);
+            } catch (...) {
+                lk.lock();
+                throw;
+            }
+            lk.lock();
+            s.busy = true;
+            ++st_.launches;
+            const auto inflight = static_cast<unsigned long long>(
+                std::count_if(slots_.begin(), slots_.end(), [](const Slot& x) { return x.busy; }));
+            st_.max_inflight_launches = std::max(st_.max_inflight_launches, inflight);
+            any = true;
+        }
+        return any;
+    }
+
+    // Pack, decode and copy back one launch on the slot's stream (lock not held).
+    void issue(Slot& s, bool wait_admit)
+    {
+        {
+            if (wait_admit) SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
             std::vector<std::vector<Job*>> by(members_.size());
             for (auto& j : s.jobs) by[j->member].push_back(j.get());
             size_t res = 0;
@@ -460,14 +514,7 @@ private:
                 res += bytes;
             }
             SICC_HIP(hipEventRecord(s.done, s.stream));
-            s.busy = true;
-            ++st_.launches;
-            const auto inflight = static_cast<unsigned long long>(
-                std::count_if(slots_.begin(), slots_.end(), [](const Slot& x) { return x.busy; }));
-            st_.max_inflight_launches = std::max(st_.max_inflight_launches, inflight);
-            any = true;
         }
-        return any;
     }
 
     std::vector<std::shared_ptr<Decoder>> members_;
@@ -488,6 +535,8 @@ private:
     sicc_server_stats_t st_{};
     bool stop_ = false;
     bool hold_ = false;
+    std::atomic<uint64_t> arrived_{0};  // jobs submitted so far (read without the lock)
+    std::atomic<bool> wake_{false};      // stop / hold / batching change for the unlocked wait
     int min_cw_ = 0;
     double max_wait_us_ = 0.0;
     bool failed_ = false;
@@ -497,7 +546,9 @@ private:
 
 HandleTable<Server>& stable()
 {
-    static HandleTable<Server> t;
+    // never destroyed: handles still open at exit must not reach the HIP
+    // runtime from static destructors
+    static HandleTable<Server>& t = *new HandleTable<Server>;
     return t;
 }
 
