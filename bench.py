@@ -250,19 +250,28 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
                                         "max_wait_us": a.serve_max_wait_us}}
     with sa.Server(decoders, budget_bytes=16 << 30, slots=4, max_launch_codewords=1 << 16) as srv:
         srv.set_batching(a.serve_min_cw, a.serve_max_wait_us)
+
+        def jobs_per_launch(s0, s1):
+            dl = s1.launches - s0.launches
+            return round((s1.jobs_completed - s0.jobs_completed) / dl, 2) if dl else None
+
+        s0 = srv.stats()
         lat, wall = srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         peak_mbps = bits / wall / 1e6
         out["peak_mbps"] = round(peak_mbps, 2)
+        out["peak_jobs_per_launch"] = jobs_per_launch(s0, srv.stats())
         loads = []
         for frac in SERVE_FRACS:
             rate_jobs = frac * peak_mbps * 1e6 / (bits / n_jobs)  # jobs per second
             arr = np.cumsum(rng.exponential(1e6 / rate_jobs, n_jobs))
+            s0 = srv.stats()
             lat, wall = srv.drive(members, counts, ptrs, arr)
             loads.append({"offered_frac": frac, "offered_mbps": round(frac * peak_mbps, 2),
                           "achieved_mbps": round(bits / wall / 1e6, 2),
                           "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 4),
                           "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 4),
-                          "p999_ms": round(float(np.percentile(lat, 99.9)) / 1e3, 4)})
+                          "p999_ms": round(float(np.percentile(lat, 99.9)) / 1e3, 4),
+                          "jobs_per_launch": jobs_per_launch(s0, srv.stats())})
         st = srv.stats()
         out["loads"] = loads
         out["mean_jobs_per_launch"] = round(st.mean_jobs_per_launch, 2)
