@@ -35,7 +35,7 @@ def _check_llr(got, ref):
     assert agree >= 0.9999, agree
 
 
-@pytest.mark.parametrize("bits", [2, 4, 6])
+@pytest.mark.parametrize("bits", [1, 2, 4, 6])
 def test_receive_matches_oracle(bits):
     # Multipath + CFO lower the full-preamble refinement peak (0.2-0.6 here), so at
     # the reference's default threshold 0.45 some frames are not detected: the
@@ -63,6 +63,30 @@ def test_receive_matches_oracle(bits):
             _check_llr(llr[f], ref[: blocks * n])
         if thr == 0.2:
             assert (sync["start"] >= 0).all()
+
+
+@pytest.mark.parametrize("max_search", [256, 3000, 0])
+def test_sync_search_windows(max_search):
+    # 256: window and prefix sums in LDS; 3000: window in LDS, direct sums per
+    # candidate; 0 (whole row): window read through L2.  Start, peak and CFO
+    # against the oracle's find_frame over the same search range.
+    print(f"[case] OFDM sync max_search={max_search}", flush=True)
+    phy, coded, ch = _cfg2_batch(4, 300 + max_search, bits=2, snr_db=16.0)
+    span = ch.rx.shape[1]
+    rx = ch.rx
+    if max_search == 3000:  # room for the longer search: pad the rows
+        rx = np.concatenate([rx, np.zeros((rx.shape[0], 3000), np.complex64)], axis=1)
+    with sa.OfdmReceiver(phy, 1024, phy.coded_bits // 1024, max_search=max_search, noise_var=ch.noise_var_freq,
+                         threshold=0.2) as rxr:
+        sync, _ = rxr.receive_host(rx)
+    for f in range(rx.shape[0]):
+        ok, start, peak, cfo, _ = oracle.find_frame(rx[f], phy.pre_half_len, phy.preamble, min(16, phy.cp), 0.2,
+                                                    phy.fs, max_search=max_search)
+        assert sync["start"][f] == start, (f, sync["start"][f], start)
+        if ok:
+            assert sync["metric"][f] == pytest.approx(peak, rel=1e-5)
+            assert sync["cfo_hz_d"][f] == pytest.approx(cfo, rel=1e-6)
+    assert span > 0
 
 
 def test_pad_and_pilot_interpolation():
