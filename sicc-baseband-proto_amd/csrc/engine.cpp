@@ -371,7 +371,7 @@ double Engine::autotune(const float* d_llr, int num_cw, const BpParams& bp, hipS
             for (int tpb : {1, 2, 4}) {
                 if (team > 1 && tpb > 1) continue;
                 for (int wpc : {8, 16, 32})
-                    for (int sc : {0, 4}) {
+                    for (int sc : {0, 4, 32}) {
                         BpTune t;
                         t.team = team;
                         t.teams_per_block = tpb;
