@@ -401,9 +401,41 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(B
 }
 
 // The single-wave shape of codes up to 128 variables and row degree 8 without
-// damping: its vector registers fit eight waves per SIMD, but the kernel
-// arguments held in scalar registers cap residency at seven.  Capping the
-// scalar budget spills a few of them to vector lanes and admits the eighth wave.
+// damping (the (128,64) class): its vector registers fit eight waves per SIMD,
+// but the kernel arguments held in scalar registers cap residency at seven.
+// Two ways to the eighth wave:
+//  * bp_flood_kernel_lean takes only the launch's hard-decision outputs and no
+//    work-stealing cursor; with the soft outputs' code gone its arguments fit
+//    the scalar budget outright;
+//  * bp_flood_kernel_w8 (any outputs) caps the scalar budget, spilling some
+//    arguments to vector lanes.
+struct BpIoLean {
+    const float* llr;
+    int num_cw;
+    uint8_t* info_bits;
+    uint8_t* success;
+    int* iterations;
+    uint8_t* used_osd;
+    float* osd_llr;
+    int* osd_index;
+    int* osd_count;
+};
+
+__global__ __launch_bounds__(256) void bp_flood_kernel_lean(BpCodeDev code, BpParams p, BpIoLean l, int e_pad)
+{
+    BpIo io;
+    io.llr = l.llr;
+    io.num_cw = l.num_cw;
+    io.info_bits = l.info_bits;
+    io.success = l.success;
+    io.iterations = l.iterations;
+    io.used_osd = l.used_osd;
+    io.osd_llr = l.osd_llr;
+    io.osd_index = l.osd_index;
+    io.osd_count = l.osd_count;
+    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad);
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bp_flood_kernel_w8(BpCodeDev code,
                                                                                                   BpParams p,
                                                                                                   BpIo io, int e_pad)
@@ -426,18 +458,28 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
     const size_t lds = team_bytes * tpb;
     auto kern = damp ? bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>;
-    if constexpr (MPL == 1 && NPL == 2 && MAXDC == 8 && TEAM == 1)
-        if (!damp) kern = bp_flood_kernel_w8;
+    bool lean = false;
+    if constexpr (MPL == 1 && NPL == 2 && MAXDC == 8 && TEAM == 1) {
+        if (!damp) {
+            lean = !io.code_bits && !io.app && !io.distance && !io.work_ctr;
+            kern = bp_flood_kernel_w8;
+        }
+    }
+    const void* kfn = lean ? reinterpret_cast<const void*>(bp_flood_kernel_lean) : reinterpret_cast<const void*>(kern);
     // A persistent grid larger than what the GPU holds at once would leave its
     // last teams to start after the first ones finished all their work.
-    const long resident_teams = static_cast<long>(resident_blocks_per_cu(reinterpret_cast<const void*>(kern), T * tpb,
-                                                                         lds)) *
-                                device_cu_count() * tpb;
+    const long resident_teams = static_cast<long>(resident_blocks_per_cu(kfn, T * tpb, lds)) * device_cu_count() * tpb;
     long teams = io.num_cw < max_waves / TEAM ? io.num_cw : max_waves / TEAM;
     if (teams > resident_teams) teams = resident_teams;
     if (teams < 1) teams = 1;
     const int blocks = static_cast<int>((teams + tpb - 1) / tpb);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, io, e_pad);
+    if (lean) {
+        const BpIoLean l{io.llr, io.num_cw, io.info_bits, io.success, io.iterations,
+                         io.used_osd, io.osd_llr, io.osd_index, io.osd_count};
+        hipLaunchKernelGGL(bp_flood_kernel_lean, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
+    } else {
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, io, e_pad);
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("bp_flood_kernel launch: ") + hipGetErrorString(e));
 }
