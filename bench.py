@@ -711,7 +711,7 @@ def main() -> int:
         "batched_vs_plain": round(h["value"] / h["plain"]["value"], 2) if h.get("plain") else None,
         "roofline": {"bound": "hbm", "achieved": round(bp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bp_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": pmc["traffic_bytes_per_launch"] if pmc else None, "kernel": "bp_flood_kernel_w8",
+                     "traffic": pmc["traffic_bytes_per_launch"] if pmc else None, "kernel": "bp_flood_kernel_lean",
                      "kernel_ms": round(h["bp_ms"], 4), "bytes_per_launch": h["hbm_bytes"]},
         "lds_roofline": {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
                          "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "bytes_per_launch": h["lds_bytes"]},
