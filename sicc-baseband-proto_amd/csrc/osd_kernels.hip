@@ -99,6 +99,38 @@ __device__ __forceinline__ unsigned long long lane_u64(unsigned long long x, int
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// 64 x 64 bit-matrix transpose across a wavefront: on entry bit j of lane i's
+// word is M[i][j]; on return it is M[j][i].  Six stages swap the off-diagonal
+// blocks of halving size between lanes i and i ^ j.
+// Lane i ^ j's value for j < 32: ds_swizzle in bit mode (and 0x1f, xor j), an
+// immediate pattern, so no lane-address registers stay live.
+template <int J>
+__device__ __forceinline__ unsigned long long swz_xor(unsigned long long x)
+{
+    constexpr int pat = 0x1f | (J << 10);
+    const int lo = __builtin_amdgcn_ds_swizzle(static_cast<int>(static_cast<unsigned>(x)), pat);
+    const int hi = __builtin_amdgcn_ds_swizzle(static_cast<int>(static_cast<unsigned>(x >> 32)), pat);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+
+template <int J>
+__device__ __forceinline__ unsigned long long transpose_stage(unsigned long long a, int lane, unsigned long long m)
+{
+    const unsigned long long b = J == 32 ? __shfl_xor(a, 32) : swz_xor<J & 31>(a);
+    return (lane & J) == 0 ? (a & m) | ((b & m) << J) : (a & ~m) | ((b & ~m) >> J);
+}
+
+__device__ __forceinline__ unsigned long long transpose64(unsigned long long a, int lane)
+{
+    a = transpose_stage<32>(a, lane, 0x00000000FFFFFFFFull);
+    a = transpose_stage<16>(a, lane, 0x0000FFFF0000FFFFull);
+    a = transpose_stage<8>(a, lane, 0x00FF00FF00FF00FFull);
+    a = transpose_stage<4>(a, lane, 0x0F0F0F0F0F0F0F0Full);
+    a = transpose_stage<2>(a, lane, 0x3333333333333333ull);
+    a = transpose_stage<1>(a, lane, 0x5555555555555555ull);
+    return a;
+}
+
 constexpr int kMaxOrder = 3;
 constexpr int kOsdWavesPerBlock = 4;
 
@@ -313,12 +345,16 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
             for (int q = 0; q < KW; ++q) pb ^= __popcll(base[q] & col[s][q]) & 1;
             d0[s] = __ballot(par && ((pb != 0) != hs[s]));
         }
-        for (int i = 0; i < k; ++i) {
+        // parity rows by position: each 64 x 64 block (lane = position, bit = row)
+        // transposed across the wave so that lane = row, bit = position
 #pragma unroll
-            for (int s = 0; s < CPL; ++s) {
-                const int c = lane + 64 * s;
-                const unsigned long long b = __ballot(c >= k && c < n && colbit<KW>(col[s], i));
-                if (lane == 0) sh.rows[i * CPL + s] = b;
+        for (int s = 0; s < CPL; ++s) {
+            const int c = lane + 64 * s;
+#pragma unroll
+            for (int q = 0; q < KW; ++q) {
+                const unsigned long long tr = transpose64(c >= k && c < n ? col[s][q] : 0ull, lane);
+                const int r = 64 * q + lane;
+                if (r < k) sh.rows[r * CPL + s] = tr;
             }
         }
         wave_lds_sync();
