@@ -81,6 +81,46 @@ def test_hard_output_launch_matches_oracle(ccsds, osd_order):
         assert np.array_equal(info, o["info"])
 
 
+def test_full_headline_batch_properties(ccsds):
+    # The headline workload at full size (1 048 576 codewords, Eb/N0 4.0 dB, BP 50
+    # iterations + order-1 OSD): every output is a codeword (H c = 0 for BP
+    # successes and for OSD results alike), the hard-output launch agrees with the
+    # full-output launch on every codeword, FER <= 1e-3, and a seeded sample is
+    # bit-exact against the oracle.
+    import torch
+
+    print("[case] full (128,64) batch, 1048576 codewords", flush=True)
+    cw = 1 << 20
+    rng = np.random.default_rng(4242)
+    info = rng.integers(0, 2, (cw, ccsds.k), dtype=np.uint8)
+    llr = oracle.awgn_llr(ccsds.encode(info), 4.0, 0.5, rng)
+    dev = torch.device("cuda:0")
+    d_llr = torch.from_numpy(llr).to(dev)
+    d_code = torch.zeros((cw, ccsds.n), dtype=torch.uint8, device=dev)
+    d_info = torch.zeros((cw, ccsds.k), dtype=torch.uint8, device=dev)
+    d_info2 = torch.zeros((cw, ccsds.k), dtype=torch.uint8, device=dev)
+    d_ok = torch.zeros(cw, dtype=torch.uint8, device=dev)
+    d_used = torch.zeros(cw, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    with sa.Decoder.stream(sa.stream_config(max_iter=50, alpha=0.75)) as dec:
+        dec.set_osd(1, 64, 4096)
+        dec.decode_device(d_llr.data_ptr(), cw, d_info=d_info.data_ptr(), d_code=d_code.data_ptr(),
+                          d_ok=d_ok.data_ptr(), d_used=d_used.data_ptr(), stream=stream)
+        dec.decode_device(d_llr.data_ptr(), cw, d_info=d_info2.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+    h = torch.from_numpy(ccsds.dense_h().astype(np.float32)).to(dev)
+    synd = torch.remainder(d_code.float() @ h.t(), 2.0)
+    assert int(synd.abs().sum().item()) == 0
+    assert torch.equal(d_info, d_info2)
+    errors = int((d_info.cpu() != torch.from_numpy(info)).any(dim=1).sum().item())
+    assert errors / cw <= 1e-3, errors
+    assert int(d_used.sum().item()) > 1000
+    sample = 20000
+    o = ccsds.bposd(llr[:sample], max_iter=50, alpha=0.75, order=1, max_flip=64, max_tests=4096, threads=8)
+    assert np.array_equal(d_code[:sample].cpu().numpy(), o["code"])
+    assert np.array_equal(d_used[:sample].cpu().numpy(), o["used_osd"])
+
+
 @pytest.mark.parametrize("order,flips", [(1, 64), (2, 32), (2, 64)])
 def test_bposd_bit_exact(ccsds, order, flips):
     print(f"[case] BP-OSD order={order} flips={flips}", flush=True)
