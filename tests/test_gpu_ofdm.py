@@ -89,6 +89,45 @@ def test_sync_search_windows(max_search):
     assert span > 0
 
 
+def test_full_cfg2_batch_properties():
+    # The receive workload at full size: 4 096 frames of 1024-point, 600-subcarrier,
+    # 16-QAM OFDM through the 6-tap fading channel with CFO and timing offsets, at
+    # 14 dB.  Nearly every frame is found at threshold 0.2, every found start is
+    # within 6 samples of the true offset, the LLR signs carry the coded bits
+    # (raw bit error rate), and a spread sample of frames matches the oracle.
+    print("[case] OFDM full batch, 4096 frames", flush=True)
+    phy = wf.OfdmPhy(bits_per_symbol=4, active_sc=600, num_symbols=14, pilot_symbols=[0])
+    rng = np.random.default_rng(2025)
+    F = 4096
+    span = phy.frame_len + 256
+    rx = np.empty((F, span), np.complex64)
+    coded = rng.integers(0, 2, (F, phy.coded_bits), dtype=np.uint8)
+    offsets = np.empty(F, np.int64)
+    nv = 0.0
+    for s0 in range(0, F, 512):
+        ch = wf.channel(phy, wf.tx_frames(phy, coded[s0:s0 + 512]), rng, span=span, snr_db=14.0)
+        rx[s0:s0 + 512] = ch.rx
+        offsets[s0:s0 + 512] = ch.offsets
+        nv = ch.noise_var_freq
+    n = 1024
+    blocks = phy.coded_bits // n
+    with sa.OfdmReceiver(phy, n, blocks, max_search=256, noise_var=nv, threshold=0.2) as rxr:
+        sync, llr = rxr.receive_host(rx)
+    found = sync["start"] >= 0
+    assert found.mean() >= 0.99, found.mean()
+    assert np.all(np.abs(sync["start"][found] - offsets[found]) <= 6)
+    # deep fades of the 6-tap channel leave ~10 % raw errors at 14 dB (the CPU oracle
+    # chain gives the same); a sign or bit-order mistake would give ~50 %
+    raw = np.mean((llr[found] < 0) != coded[found, : blocks * n])
+    assert raw < 0.2, raw
+    for f in range(0, F, 256):
+        if not found[f]:
+            continue
+        s = sync["start"][f]
+        ref = oracle.demod_frame(rx[f][s:s + phy.frame_len], phy, sync["cfo_hz_d"][f], nv)
+        _check_llr(llr[f], ref[: blocks * n])
+
+
 def test_pad_and_pilot_interpolation():
     # two pilots with data symbols between and after them, bits not a multiple of n
     phy = wf.OfdmPhy(bits_per_symbol=2, active_sc=300, num_symbols=9, pilot_symbols=[0, 5])
