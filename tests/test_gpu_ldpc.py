@@ -121,6 +121,37 @@ def test_full_headline_batch_properties(ccsds):
     assert np.array_equal(d_used[:sample].cpu().numpy(), o["used_osd"])
 
 
+def test_full_order2_osd_batch_properties(ccsds):
+    # The OSD-heavy workload at full size: 262 144 frames at Eb/N0 1.0 dB, 20 BP
+    # iterations, every failure to order-2 OSD (2 081 test patterns): all outputs
+    # are codewords, about three quarters of the frames go through OSD, and a seeded
+    # sample is bit-exact against the oracle.
+    import torch
+
+    print("[case] full order-2 OSD batch, 262144 frames", flush=True)
+    cw = 1 << 18
+    rng = np.random.default_rng(777)
+    info = rng.integers(0, 2, (cw, ccsds.k), dtype=np.uint8)
+    llr = oracle.awgn_llr(ccsds.encode(info), 1.0, 0.5, rng)
+    dev = torch.device("cuda:0")
+    d_llr = torch.from_numpy(llr).to(dev)
+    d_code = torch.zeros((cw, ccsds.n), dtype=torch.uint8, device=dev)
+    d_used = torch.zeros(cw, dtype=torch.uint8, device=dev)
+    with sa.Decoder.stream(sa.stream_config(max_iter=20, alpha=0.75)) as dec:
+        dec.set_osd(2, 64, 4096)
+        dec.decode_device(d_llr.data_ptr(), cw, d_code=d_code.data_ptr(), d_used=d_used.data_ptr(),
+                          stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+    h = torch.from_numpy(ccsds.dense_h().astype(np.float32)).to(dev)
+    assert int(torch.remainder(d_code.float() @ h.t(), 2.0).abs().sum().item()) == 0
+    used = d_used.cpu().numpy()
+    assert 0.7 < used.mean() < 0.85, used.mean()  # BP fails on ~3/4 of the frames at 1.0 dB
+    sample = 3000
+    o = ccsds.bposd(llr[:sample], max_iter=20, alpha=0.75, order=2, max_flip=64, max_tests=4096, threads=8)
+    assert np.array_equal(d_code[:sample].cpu().numpy(), o["code"])
+    assert np.array_equal(used[:sample], o["used_osd"])
+
+
 @pytest.mark.parametrize("order,flips", [(1, 64), (2, 32), (2, 64)])
 def test_bposd_bit_exact(ccsds, order, flips):
     print(f"[case] BP-OSD order={order} flips={flips}", flush=True)
