@@ -88,3 +88,52 @@ def test_graph_recaptured_when_groups_change():
                 assert np.array_equal(info[i], o["hard"][code.info_cols])
     for dd in decs:
         dd.close()
+
+
+def test_full_mixed_batch():
+    # The mixed workload at full size: 65 536 frames drawn uniformly over 15 QC
+    # codes (n 128..2048, 8 / 5 / 4 block rows), decoded by one graph launch:
+    # replay equal to eager on every frame, and a spread sample of every code
+    # bit-exact against the oracle.
+    import torch
+
+    print("[case] full mixed batch, 65536 frames over 15 codes", flush=True)
+    rng = np.random.default_rng(65536)
+    shapes = [(n, mb) for n in (128, 256, 512, 1024, 2048) for mb in (8, 5, 4)]
+    member = rng.integers(0, len(shapes), 65536)
+    decs, codes, data = [], [], []
+    for j, (n, mb) in enumerate(shapes):
+        cnt = int((member == j).sum())
+        m, rp, ci = sa.qc_code(n, mb, 3, seed=1000 + n + mb)
+        code = oracle.Code(n, m, rp, ci, front_info=True)
+        info = rng.integers(0, 2, (cnt, code.k), dtype=np.uint8)
+        llr = oracle.awgn_llr(code.encode(info), 3.0, code.k / n, rng)
+        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=50, alpha=0.75, schedule=0))
+        codes.append(code)
+        data.append({"llr": llr, "d_llr": torch.from_numpy(llr).cuda(),
+                     "d_info": torch.zeros((cnt, code.k), dtype=torch.uint8, device="cuda"),
+                     "d_ok": torch.zeros(cnt, dtype=torch.uint8, device="cuda"),
+                     "d_it": torch.zeros(cnt, dtype=torch.int32, device="cuda")})
+    assert sum(d["llr"].shape[0] for d in data) == 65536
+    with sa.MultiDecoder(decs) as md:
+        md.decode(_groups(data), use_graph=False)
+        torch.cuda.synchronize()
+        eager = [tuple(d[k].cpu().numpy().copy() for k in ("d_info", "d_ok", "d_it")) for d in data]
+        for d in data:
+            d["d_info"].zero_()
+            d["d_ok"].fill_(7)
+            d["d_it"].fill_(-1)
+        torch.cuda.synchronize()
+        md.decode(_groups(data), use_graph=True)
+        torch.cuda.synchronize()
+        for d, (info, ok, it) in zip(data, eager):
+            assert np.array_equal(d["d_info"].cpu().numpy(), info)
+            assert np.array_equal(d["d_ok"].cpu().numpy(), ok)
+            assert np.array_equal(d["d_it"].cpu().numpy(), it)
+    for code, d, (info, ok, it) in zip(codes, data, eager):
+        for i in range(0, d["llr"].shape[0], 97):
+            o = code.nms(d["llr"][i], max_iter=50, alpha=0.75)
+            assert ok[i] == o["ok"] and it[i] == o["iterations"], (code.n, i)
+            assert np.array_equal(info[i], o["hard"][code.info_cols]), (code.n, i)
+    for dd in decs:
+        dd.close()
