@@ -18,7 +18,7 @@ import waveform as wf
 pytestmark = pytest.mark.gpu
 
 
-def _cfg2_batch(frames, seed, bits=4, snr_db=14.0, fading=True, num_symbols=14, active_sc=600):
+def _ofdm_batch(frames, seed, bits=4, snr_db=14.0, fading=True, num_symbols=14, active_sc=600):
     phy = wf.OfdmPhy(bits_per_symbol=bits, active_sc=active_sc, num_symbols=num_symbols)
     rng = np.random.default_rng(seed)
     coded = rng.integers(0, 2, (frames, phy.coded_bits), dtype=np.uint8)
@@ -42,7 +42,7 @@ def test_receive_matches_oracle(bits):
     # found / not-found pattern must match the oracle exactly.  LLRs are compared
     # at threshold 0.2 (the reference's --sync-threshold option), where all are found.
     print(f"[case] OFDM receive bits={bits}", flush=True)
-    phy, coded, ch = _cfg2_batch(24, 100 + bits, bits=bits, snr_db=14.0 + 6 * (bits == 6))
+    phy, coded, ch = _ofdm_batch(24, 100 + bits, bits=bits, snr_db=14.0 + 6 * (bits == 6))
     n = 1024
     blocks = phy.coded_bits // n
     for thr in (0.45, 0.2):
@@ -71,7 +71,7 @@ def test_sync_search_windows(max_search):
     # candidate; 0 (whole row): window read through L2.  Start, peak and CFO
     # against the oracle's find_frame over the same search range.
     print(f"[case] OFDM sync max_search={max_search}", flush=True)
-    phy, coded, ch = _cfg2_batch(4, 300 + max_search, bits=2, snr_db=16.0)
+    phy, coded, ch = _ofdm_batch(4, 300 + max_search, bits=2, snr_db=16.0)
     span = ch.rx.shape[1]
     rx = ch.rx
     if max_search == 3000:  # room for the longer search: pad the rows
@@ -89,7 +89,7 @@ def test_sync_search_windows(max_search):
     assert span > 0
 
 
-def test_full_cfg2_batch_properties():
+def test_full_ofdm_batch_properties():
     # The receive workload at full size: 4 096 frames of 1024-point, 600-subcarrier,
     # 16-QAM OFDM through the 6-tap fading channel with CFO and timing offsets, at
     # 14 dB.  Nearly every frame is found at threshold 0.2, every found start is
@@ -160,7 +160,7 @@ def test_no_frame_gives_erasures():
 
 
 def test_find_frame_abi_matches_oracle():
-    phy, coded, ch = _cfg2_batch(3, 5, bits=2, fading=False)
+    phy, coded, ch = _ofdm_batch(3, 5, bits=2, fading=False)
     for f in range(3):
         s1, peak, cfo = sa.find_frame(ch.rx[f].astype(np.complex128), phy.preamble, phy.pre_half_len, 0.45, phy.fs)
         ok, start, opeak, ocfo, _ = oracle.find_frame(ch.rx[f], phy.pre_half_len, phy.preamble, 16, 0.45, phy.fs)

@@ -3,7 +3,7 @@
 The reference selects a "colored layered" GPU BP schedule for n >= 1000
 (source/cpp_cuda/uhd_cpp/src/uhd_ldpc_ofdm_link.cpp:3658) inside a decoder
 library that is absent from the reference tree, so the schedule's exact
-definition is this build's (greedy first-fit row colouring in row order, rows
+definition is chosen here (greedy first-fit row colouring in row order, rows
 of a colour processed as independent layers).  It is pinned here by:
   - the colouring: every colour an independent set, greedy-minimal, order
     sorted by (colour, row), recomputed independently in numpy;
