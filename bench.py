@@ -238,7 +238,8 @@ def calibrate_mix(a, sa, np, torch):
 
 
 # ------------------------------------------------------------------- served --
-SERVE_FRACS = (0.1, 0.25, 0.5, 0.75, 0.9, 1.0)
+SERVE_FRACS = (0.1, 0.25, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)
+SERVE_P99_SLO_MS = 2.0  # request-latency objective the served throughput is quoted at
 
 
 def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
@@ -255,6 +256,9 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
             dl = s1.launches - s0.launches
             return round((s1.jobs_completed - s0.jobs_completed) / dl, 2) if dl else None
 
+        # warm-up: the first launches on each slot allocate its packing and result
+        # buffers (grow-only); steady-state serving reuses them
+        srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         s0 = srv.stats()
         lat, wall = srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         peak_mbps = bits / wall / 1e6
@@ -281,6 +285,7 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
                    max_launch_codewords=1 << 16) as srv:
         m = min(n_jobs, 2000)
         mbits = float(sum(int(c) * ks[x] for x, c in zip(members[:m], counts[:m])))
+        srv.drive(members[:m], counts[:m], ptrs[:m], np.zeros(m))  # warm-up, as above
         lat, wall = srv.drive(members[:m], counts[:m], ptrs[:m], np.zeros(m))
         plain_peak = mbits / wall / 1e6
         arr = np.cumsum(rng.exponential(1e6 / (0.1 * peak_mbps * 1e6 / (bits / n_jobs)), m))
@@ -722,12 +727,17 @@ def main() -> int:
     }
     out.update(extra)
     if "served" in extra:
-        # request latency of the served (128,64) decoder at 90 % of its peak load
-        s = [x for x in extra["served"]["ccsds128"]["loads"] if x["offered_frac"] == 0.9][0]
+        # the served (128,64) decoder at the highest offered load whose p99 request
+        # latency meets the objective (the lowest load if none does)
+        loads = extra["served"]["ccsds128"]["loads"]
+        ok = [x for x in loads if x["p99_ms"] <= SERVE_P99_SLO_MS]
+        s = max(ok, key=lambda x: x["achieved_mbps"]) if ok else loads[0]
         out["p50_ms"] = s["p50_ms"]
         out["p99_ms"] = s["p99_ms"]
         out["p999_ms"] = s["p999_ms"]
         out["served_mbps_at_p99"] = s["achieved_mbps"]
+        out["served_p99_slo_ms"] = SERVE_P99_SLO_MS
+        out["served_offered_frac"] = s["offered_frac"]
     print(json.dumps(out))
     dec128.close()
     return 0
