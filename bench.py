@@ -316,7 +316,7 @@ def leg_served(a, sa, np, torch, dec128):
                         "ebn0_db": a.ebn0}}
     torch.cuda.synchronize()
     out["ccsds128"].update(serve_sweep(a, sa, np, rng, [dec128], [0] * n_jobs, counts, ptrs, [64]))
-    # config-3 mix at FER ~1e-3: one flooding BP decoder per code, LLR pools in HBM
+    # the 15-code QC mix at FER ~1e-3: one flooding BP decoder per code, LLR pools in HBM
     shapes = sorted(MIX_EBN0_SERVE)
     decs, pools, ks = [], [], []
     for n, mb in shapes:

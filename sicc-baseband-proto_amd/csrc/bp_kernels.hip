@@ -70,6 +70,32 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* tab
     }
 }
 
+// Result packing for the server's copy back: one wavefront per 64 information
+// bits of a row (ballot -> 8 bytes, bit i of a row at byte i/8, bit i%8), then
+// the trailing `flag_bytes` bytes (per-codeword flags) copied behind the packed
+// rows, one byte per thread in the blocks after the packing ones.
+__global__ __launch_bounds__(256) void pack_bits_kernel(const uint8_t* __restrict__ bits, int rows, int k,
+                                                        const uint8_t* __restrict__ flags, int flag_bytes,
+                                                        uint8_t* __restrict__ out, int pack_blocks)
+{
+    const int kb = (k + 7) >> 3;
+    if (static_cast<int>(blockIdx.x) >= pack_blocks) {
+        const int i = (blockIdx.x - pack_blocks) * 256 + threadIdx.x;
+        if (i < flag_bytes) out[static_cast<size_t>(rows) * kb + i] = flags[i];
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int chunks = (k + 63) >> 6;
+    const long w = static_cast<long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (w >= static_cast<long>(rows) * chunks) return;  // whole wavefront
+    const long r = w / chunks;
+    const int c = static_cast<int>(w - r * chunks);
+    const int i = c * 64 + lane;
+    const unsigned long long m = __ballot(i < k && bits[r * k + i] != 0);
+    const int nb = min(8, kb - c * 8);
+    if (lane < nb) out[r * kb + c * 8 + lane] = static_cast<uint8_t>(m >> (8 * lane));
+}
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
 __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
@@ -520,6 +546,20 @@ void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int r
     hipLaunchKernelGGL(gather_rows_kernel, dim3(entries), dim3(256), 0, s, table, dst, row_len);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("gather_rows_kernel launch: ") + hipGetErrorString(e));
+}
+
+void launch_pack_bits(const uint8_t* bits, int rows, int k, const uint8_t* flags, int flag_bytes, uint8_t* out,
+                      hipStream_t s)
+{
+    if (rows <= 0 || k <= 0 || flag_bytes < 0) throw std::invalid_argument("pack_bits: bad shape");
+    const long waves = static_cast<long>(rows) * ((k + 63) / 64);
+    const long pack_blocks = (waves + 3) / 4;
+    const long flag_blocks = (flag_bytes + 255) / 256;
+    if (pack_blocks + flag_blocks > 0x7fffffffL) throw std::invalid_argument("pack_bits: batch too large");
+    hipLaunchKernelGGL(pack_bits_kernel, dim3(static_cast<unsigned>(pack_blocks + flag_blocks)), dim3(256), 0, s, bits,
+                       rows, k, flags, flag_bytes, out, static_cast<int>(pack_blocks));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("pack_bits_kernel launch: ") + hipGetErrorString(e));
 }
 
 bool bp_team_supported(const BpCodeDev& c, int team)

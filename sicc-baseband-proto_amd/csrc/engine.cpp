@@ -213,12 +213,13 @@ BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool
     return ls;
 }
 
-void Engine::reserve(int num_cw, const BpParams* bp)
+void Engine::reserve(int num_cw, const BpParams* bp, DecodeScratch* sc)
 {
-    own_.ensure(num_cw, code_.n);
+    DecodeScratch& s = sc ? *sc : own_;
+    s.ensure(num_cw, code_.n);
     if (bp && uses_long_kernel(*bp)) {
         const bool layered = bp->schedule >= 2 || (bp->schedule < 0 && code_.n >= 1000);
-        long_scratch(long_tables(layered), *bp, layered, own_);
+        long_scratch(long_tables(layered), *bp, layered, s);
     }
 }
 

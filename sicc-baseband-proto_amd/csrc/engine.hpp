@@ -65,8 +65,10 @@ public:
     int max_waves() const { return max_waves_; }
     // Pre-size the OSD hand-off scratch for batches of up to num_cw codewords,
     // and the long-code scratch when `bp` selects that kernel (decode() grows
-    // them on demand, which a graph capture cannot do).
-    void reserve(int num_cw, const BpParams* bp = nullptr);
+    // them on demand, which a graph capture cannot do and which costs a
+    // device-wide hipFree in a serving loop).  `sc` names a caller-owned
+    // scratch (as passed to decode()); null means the engine's own.
+    void reserve(int num_cw, const BpParams* bp = nullptr, DecodeScratch* sc = nullptr);
     // True when `bp` runs the long-code kernel (layered schedule or a code the
     // short-code tables cannot hold).
     bool uses_long_kernel(const BpParams& bp) const;

@@ -94,6 +94,10 @@ struct GatherEntry {
     int dst_row;
 };
 void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int row_len, hipStream_t s);
+// Bit-pack `rows` rows of k 0/1 bytes into ceil(k/8) bytes each (bit i at byte
+// i/8, bit i%8) and append `flag_bytes` bytes from `flags` behind them.
+void launch_pack_bits(const uint8_t* bits, int rows, int k, const uint8_t* flags, int flag_bytes, uint8_t* out,
+                      hipStream_t s);
 
 // Kernel launchers.  They validate shapes on the host and throw
 // std::invalid_argument when a code does not fit an instantiated variant.

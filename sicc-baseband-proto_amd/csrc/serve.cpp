@@ -90,7 +90,7 @@ struct Job {
     size_t grp_base = 0;           // this job's group block in the slot's pinned results
     int row = 0;                   // first row of the job inside its group
     int grp_cw = 0;                // rows in the group
-    std::vector<uint8_t> info, ok, used;
+    std::vector<uint8_t> info, ok, used;  // info bit-packed: ceil(k/8) bytes per codeword
 };
 
 struct Slot {
@@ -101,7 +101,7 @@ struct Slot {
     std::vector<DevBuf<float>> pack_llr;    // per member, max_cw rows
     std::vector<DevBuf<uint8_t>> pack_out;  // per member, [info | ok | used] of one launch
     std::vector<DecodeScratch> scratch;
-    uint8_t* h_res = nullptr;  // pinned results of one launch
+    uint8_t* h_res = nullptr;  // pinned results of one launch (bit-packed info), written by the GPU
     size_t h_res_bytes = 0;
     GatherEntry* table = nullptr;  // pinned gather table of one launch
     int table_cap = 0;
@@ -116,8 +116,8 @@ public:
         SICC_HIP(hipMalloc(reinterpret_cast<void**>(&base_), budget));
         SICC_HIP(hipStreamCreateWithFlags(&admit_stream_, hipStreamNonBlocking));
         SICC_HIP(hipEventCreateWithFlags(&admit_event_, hipEventDisableTiming));
-        size_t kmax = 0;
-        for (auto& m : members_) kmax = std::max<size_t>(kmax, m->eng->code().k);
+        size_t kbmax = 0;
+        for (auto& m : members_) kbmax = std::max<size_t>(kbmax, (m->eng->code().k + 7) / 8);
         slots_.resize(std::max(slots, 1));
         for (auto& s : slots_) {
             SICC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
@@ -125,11 +125,22 @@ public:
             s.pack_llr.resize(members_.size());
             s.pack_out.resize(members_.size());
             s.scratch.resize(members_.size());
-            s.h_res_bytes = static_cast<size_t>(max_cw_) * (kmax + 2);
+            s.h_res_bytes = static_cast<size_t>(max_cw_) * (kbmax + 2);
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_res), s.h_res_bytes, hipHostMallocDefault));
             s.table_cap = max_cw_;  // at most one entry per codeword
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.table), sizeof(GatherEntry) * s.table_cap,
                                    hipHostMallocDefault));
+            // Launch staging at its largest size up front: a buffer grown in the
+            // serving loop would hipFree (a device-wide wait behind every launch in
+            // flight) and stall the pipeline just when launches get large.
+            for (size_t m = 0; m < members_.size(); ++m) {
+                Decoder& d = *members_[m];
+                const LdpcCode& c = d.eng->code();
+                const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
+                s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
+                s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));
+                d.eng->reserve(osd >= 0 ? max_cw_ : 0, &d.bp, &s.scratch[m]);
+            }
         }
         thread_ = std::thread([this] { loop(); });
     }
@@ -221,7 +232,16 @@ public:
             j = it->second;
             known_.erase(it);
         }
-        if (info) std::memcpy(info, j->info.data(), j->info.size());
+        if (info) {
+            // unpack: bit i of a codeword sits at byte i/8, bit i%8
+            const int k = members_[j->member]->eng->code().k;
+            const size_t kb = (static_cast<size_t>(k) + 7) / 8;
+            for (int r = 0; r < j->count; ++r) {
+                const uint8_t* src = j->info.data() + r * kb;
+                uint8_t* dst = info + static_cast<size_t>(r) * k;
+                for (int i = 0; i < k; ++i) dst[i] = (src[i >> 3] >> (i & 7)) & 1u;
+            }
+        }
         if (ok) std::memcpy(ok, j->ok.data(), j->ok.size());
         if (used) std::memcpy(used, j->used.data(), j->used.size());
         if (lat) *lat = static_cast<double>(j->t_done - j->t_submit);
@@ -296,7 +316,7 @@ private:
             SICC_HIP(hipSetDevice(device_));
             std::unique_lock<std::mutex> lk(mu_);
             while (!stop_) {
-                bool progress = harvest();
+                bool progress = harvest(lk);
                 if (!hold_) {
                     progress |= admit();
                     progress |= launch(lk);
@@ -343,23 +363,37 @@ private:
         }
     }
 
-    bool harvest()
+    // Completed launches: results leave the slot's pinned buffer with the lock
+    // released (only this thread touches slots and in-flight jobs), then the
+    // jobs are published under the lock.
+    bool harvest(std::unique_lock<std::mutex>& lk)
     {
-        bool any = false;
+        done_.clear();
         for (auto& s : slots_) {
             if (!s.busy) continue;
             const hipError_t q = hipEventQuery(s.done);
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess) throw_hip(q, "server launch");
-            const uint64_t t = now_us();
-            for (auto& j : s.jobs) {
-                const int k = members_[j->member]->eng->code().k;
+            done_.push_back(&s);
+        }
+        if (done_.empty()) return false;
+        lk.unlock();
+        const uint64_t t = now_us();
+        for (Slot* s : done_) {
+            for (auto& j : s->jobs) {
+                const size_t kb = (static_cast<size_t>(members_[j->member]->eng->code().k) + 7) / 8;
                 const size_t n = static_cast<size_t>(j->count);
-                const uint8_t* g = s.h_res + j->grp_base;
+                const size_t row = static_cast<size_t>(j->row);
                 const size_t gcw = static_cast<size_t>(j->grp_cw);
-                j->info.assign(g + static_cast<size_t>(j->row) * k, g + (static_cast<size_t>(j->row) + n) * k);
-                j->ok.assign(g + gcw * k + j->row, g + gcw * k + j->row + n);
-                j->used.assign(g + gcw * (k + 1) + j->row, g + gcw * (k + 1) + j->row + n);
+                const uint8_t* g = s->h_res + j->grp_base;
+                j->info.assign(g + row * kb, g + (row + n) * kb);
+                j->ok.assign(g + gcw * kb + row, g + gcw * kb + row + n);
+                j->used.assign(g + gcw * (kb + 1) + row, g + gcw * (kb + 1) + row + n);
+            }
+        }
+        lk.lock();
+        for (Slot* s : done_) {
+            for (auto& j : s->jobs) {
                 j->t_done = t;
                 pool_.release(j->off, j->bytes);
                 j->host_copy = std::vector<float>();
@@ -367,12 +401,11 @@ private:
                 ++st_.jobs_completed;
                 st_.codewords_completed += static_cast<unsigned long long>(j->count);
             }
-            s.jobs.clear();
-            s.busy = false;
-            any = true;
+            s->jobs.clear();
+            s->busy = false;
         }
-        if (any) cv_done_.notify_all();
-        return any;
+        cv_done_.notify_all();
+        return true;
     }
 
     bool admit()
@@ -485,7 +518,6 @@ private:
                 if (by[m].size() == 1) {
                     llr = by[m][0]->d_llr;
                 } else {
-                    if (s.pack_llr[m].size() == 0) s.pack_llr[m].alloc(static_cast<size_t>(max_cw_) * c.n);
                     // one gather workgroup per chunk of about 4096 LLRs, so a few
                     // large jobs still spread over the whole GPU
                     const int ch = std::max(1, 4096 / c.n);
@@ -498,15 +530,18 @@ private:
                     tbl += ne;
                     llr = s.pack_llr[m].get();
                 }
-                if (s.pack_out[m].size() == 0) s.pack_out[m].alloc(static_cast<size_t>(max_cw_) * (c.k + 2));
                 uint8_t* o = s.pack_out[m].get();
                 DecodeOut out;
                 out.info = o;
                 out.bp_ok = o + static_cast<size_t>(cw) * c.k;
                 out.used_osd = o + static_cast<size_t>(cw) * (c.k + 1);
                 d.eng->decode(llr, cw, d.bp, osd, d.osd, out, s.stream, &s.scratch[m]);
-                const size_t bytes = static_cast<size_t>(cw) * (c.k + 2);
-                SICC_HIP(hipMemcpyAsync(s.h_res + res, o, bytes, hipMemcpyDeviceToHost, s.stream));
+                // bit-pack the information bits (8x less to move and to read on the
+                // host) with the ok / used flags behind them, written by the kernel
+                // straight into the slot's pinned results: no copy-engine transfer,
+                // whose D2H submissions stalled the host for milliseconds at times
+                const size_t bytes = static_cast<size_t>(cw) * ((c.k + 7) / 8 + 2);
+                launch_pack_bits(o, cw, c.k, out.bp_ok, 2 * cw, s.h_res + res, s.stream);
                 for (Job* j : by[m]) {
                     j->grp_base = res;
                     j->grp_cw = cw;
@@ -527,6 +562,7 @@ private:
     hipEvent_t admit_event_ = nullptr;
     bool admit_pending_ = false;
     std::vector<Slot> slots_;
+    std::vector<Slot*> done_;  // harvest's completed slots
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
     std::deque<std::shared_ptr<Job>> arrivals_, resident_;

@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 import sicc_amd as sa
+from ldpc_helpers import gf2_rank
 
 pytestmark = pytest.mark.gpu
 
@@ -124,3 +125,39 @@ def test_native_driver_latencies():
             srv.set_batching(-1, 0.0)
     for d in decs:
         d.close()
+
+
+def test_ragged_info_width_round_trip():
+    """k = 73 (neither a multiple of 8 nor of 64): results travel bit-packed from
+    the GPU and are unpacked by fetch; every codeword equals the oracle."""
+    print("[case] server: k=73 code, packed results", flush=True)
+    rng = np.random.default_rng(11)
+    n, m = 140, 67
+    while True:  # column weight 3, row degree <= 16, last m columns invertible
+        h = np.zeros((m, n), np.uint8)
+        for c in range(n):
+            h[rng.choice(m, 3, replace=False), c] = 1
+        if h.sum(axis=1).max() <= 16 and h.sum(axis=1).min() >= 2 and gf2_rank(h[:, n - m:]) == m:
+            break
+    rp = np.concatenate([[0], np.cumsum(h.sum(axis=1))]).astype(np.int32)
+    ci = np.concatenate([np.nonzero(r)[0] for r in h]).astype(np.int32)
+    code = oracle.Code(n, m, rp, ci, True)
+    assert code.k == 73
+    dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0)
+    jobs = []
+    for _ in range(12):
+        info = rng.integers(0, 2, (int(rng.integers(1, 40)), code.k), dtype=np.uint8)
+        jobs.append(oracle.awgn_llr(code.encode(info), 2.5, code.k / n, rng))
+    with sa.Server([dec], budget_bytes=64 << 20, slots=2) as srv:
+        srv.hold(True)  # one packed launch of all twelve jobs
+        for j, llr in enumerate(jobs):
+            srv.submit(j, 0, llr)
+        srv.hold(False)
+        srv.drain()
+        for j, llr in enumerate(jobs):
+            info, ok, used, _ = srv.fetch(j, llr.shape[0], code.k)
+            for i in range(llr.shape[0]):
+                o = code.nms(llr[i], max_iter=30, alpha=0.75)
+                assert np.array_equal(info[i], o["hard"][code.info_cols]), (j, i)
+                assert bool(ok[i]) == o["ok"] and not used[i], (j, i)
+    dec.close()
