@@ -96,6 +96,19 @@ __global__ __launch_bounds__(256) void pack_bits_kernel(const uint8_t* __restric
     if (lane < nb) out[r * kb + c * 8 + lane] = static_cast<uint8_t>(m >> (8 * lane));
 }
 
+// XOR of N words with three-input XORs (v_bitop3_b32, truth table 0x96):
+// N - 1 two-input XORs become ceil((N - 1) / 2) instructions.
+template <int N>
+__device__ __forceinline__ unsigned xor_reduce(const unsigned (&v)[N])
+{
+    unsigned x = v[0];
+    int j = 1;
+#pragma unroll
+    for (; j + 1 < N; j += 2) x = __builtin_amdgcn_bitop3_b32(x, v[j], v[j + 1], 0x96);
+    if (j < N) x ^= v[j];
+    return x;
+}
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
 __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
@@ -227,12 +240,12 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
             unsigned sps[MPL];
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
-                unsigned hp = 0u, sp = 0u;
+                unsigned hb[MAXDC], qb[MAXDC];
                 float m1 = FLT_MAX, m2 = FLT_MAX;
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
                     const float a = app[em[s][j] >> 16];
-                    hp ^= __float_as_uint(a);  // canonical APP: sign bit == (APP < 0)
+                    hb[j] = __float_as_uint(a);  // canonical APP: sign bit == (APP < 0)
                     // The LDS APP is never -0, so a - r_old is never -0 either (x - y is
                     // -0 only for x = -0, y = +0): q's sign bit is (q < 0) as in the
                     // reference, whose q can differ only in the sign of a zero.  Damped
@@ -244,16 +257,16 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                         q = q + 0.0f;
                     }
                     qv[s][j] = q;
-                    sp ^= __float_as_uint(q);
+                    qb[j] = __float_as_uint(q);
                     const float av = fabsf(q);
                     // m1 <= m2 always, so min(m2, max(m1, av)) is the median of the three
                     m2 = __builtin_amdgcn_fmed3f(m1, m2, av);
                     m1 = fminf(m1, av);
                 }
-                synw += __popcll(__ballot((hp >> 31) != 0u));
+                synw += __popcll(__ballot((xor_reduce(hb) >> 31) != 0u));
                 m1s[s] = m1;
                 m2s[s] = m2;
-                sps[s] = sp;
+                sps[s] = xor_reduce(qb);
             }
             if (TEAM > 1) {
                 // team-wide syndrome weight through an LDS counter, double-buffered by iteration
@@ -290,11 +303,17 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                 // keep the two products per row (not one per edge after the select)
                 float b1 = a1, b2 = a2;
                 asm volatile("" : "+v"(b1), "+v"(b2));
+                // the row's sign product folded into both magnitudes once (they are
+                // >= +0, so XOR of a sign bit is OR); per edge only q's own sign is
+                // XORed in: one three-input op (v_bitop3) instead of XOR + AND-OR
+                const unsigned sg = sps[s] & 0x80000000u;
+                unsigned b1s = __float_as_uint(b1) ^ sg;
+                unsigned b2s = __float_as_uint(b2) ^ sg;
+                asm volatile("" : "+v"(b1s), "+v"(b2s));  // keep the fold (not re-split per edge)
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
-                    const float mag = (fabsf(qv[s][j]) == m1) ? b2 : b1;
-                    const float r =
-                        __uint_as_float(__float_as_uint(mag) | ((sps[s] ^ __float_as_uint(qv[s][j])) & 0x80000000u));
+                    const unsigned mag = (fabsf(qv[s][j]) == m1) ? b2s : b1s;
+                    const float r = __uint_as_float(mag ^ (__float_as_uint(qv[s][j]) & 0x80000000u));
                     rold[s][j] = r;
                     rcol[em[s][j] & 0xffff] = r;
                 }
