@@ -148,14 +148,14 @@ struct OsdLds {
 template <int KW>
 __device__ __forceinline__ bool colbit(const unsigned long long* w, int r)
 {
-    return ((w[r >> 6] >> (r & 63)) & 1ull) != 0ull;
+    return ((w[KW == 1 ? 0 : r >> 6] >> (r & 63)) & 1ull) != 0ull;
 }
 
 // Any one in rows >= t.
 template <int KW>
 __device__ __forceinline__ bool rows_from(const unsigned long long* w, int t)
 {
-    const int tw = t >> 6;
+    const int tw = KW == 1 ? 0 : t >> 6;  // t < k <= 64 KW
     bool any = false;
 #pragma unroll
     for (int q = 0; q < KW; ++q)
@@ -219,7 +219,9 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
         }
         int status = 0;
         for (int t = 0; t < k; ++t) {
-            const int tw = t >> 6;
+            // t < k <= 64 KW: with one row word the word (and the column slot of
+            // column t) is 0, which the compiler cannot infer on its own
+            const int tw = KW == 1 ? 0 : t >> 6;
             const unsigned long long tbit = 1ull << (t & 63);
             int pc = -1;
 #pragma unroll
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 if (m != 0ull) prow = 64 * q + __ffsll(static_cast<long long>(m)) - 1;
             }
             if (prow != t) {
-                const int pw = prow >> 6;
+                const int pw = KW == 1 ? 0 : prow >> 6;
                 const unsigned long long pbit = 1ull << (prow & 63);
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
                 for (int q = 0; q < KW; ++q) wt[q] = wp[q] = 0ull;
 #pragma unroll
                 for (int s = 0; s < CPL; ++s) {
-                    if (s == (t >> 6)) {
+                    if (s == tw) {
 #pragma unroll
                         for (int q = 0; q < KW; ++q) wt[q] = lane_u64(col[s][q], t & 63);
                         st = __builtin_amdgcn_readlane(sp[s], t & 63);
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(256) void osd_kernel(OsdCodeDev code, OsdParams p, 
             for (int s = 0; s < CPL; ++s)
 #pragma unroll
                 for (int q = 0; q < KW; ++q)
-                    if (s == (t >> 6)) clr[q] = lane_u64(col[s][q], t & 63);
+                    if (s == tw) clr[q] = lane_u64(col[s][q], t & 63);
             clr[tw] &= ~tbit;
 #pragma unroll
             for (int s = 0; s < CPL; ++s)
