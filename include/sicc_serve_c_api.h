@@ -63,8 +63,10 @@ int sicc_server_hold(unsigned long long handle, int hold);
 
 /* Native load driver: submits job j at arrival_us[j] (microseconds after the
  * call starts, busy-waiting on a steady clock) from device LLR sources, waits
- * for all, and returns each job's latency and the wall time.  Results are
- * discarded.  For benchmarks. */
+ * for all, and returns each job's latency and the wall time.  Jobs already due
+ * when the driver gets to them are queued together (up to 256 under one lock),
+ * as a client holding a backlog would.  Results are discarded.  For
+ * benchmarks. */
 int sicc_server_drive(unsigned long long handle, int n_jobs, const int* member, const int* count,
                       const unsigned long long* d_llr_ptrs, const double* arrival_us, double* latency_us,
                       double* wall_s);

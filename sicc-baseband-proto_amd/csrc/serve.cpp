@@ -180,7 +180,8 @@ public:
         return b;
     }
 
-    void submit(uint64_t id, int member, const float* llr, int count, bool dev)
+    // A validated job, not yet visible to the scheduler (no lock taken).
+    std::shared_ptr<Job> make_job(uint64_t id, int member, const float* llr, int count, bool dev)
     {
         if (member < 0 || member >= static_cast<int>(members_.size())) throw std::invalid_argument("submit: bad member");
         if (!llr || count <= 0) throw std::invalid_argument("submit: empty job");
@@ -200,15 +201,35 @@ public:
             j->src = j->host_copy.data();
         }
         j->t_submit = now_us();
+        return j;
+    }
+
+    // Queue jobs under one lock acquisition (all or none: a duplicate id rejects
+    // the group).
+    void enqueue(const std::shared_ptr<Job>* jobs, size_t count)
+    {
+        if (count == 0) return;
         {
             std::lock_guard<std::mutex> lk(mu_);
-            if (known_.count(id)) throw std::invalid_argument("submit: job id already in use");
-            known_[id] = j;
-            arrivals_.push_back(j);
-            ++st_.jobs_submitted;
-            arrived_.fetch_add(1, std::memory_order_release);
+            for (size_t i = 0; i < count; ++i) {
+                if (known_.count(jobs[i]->id)) throw std::invalid_argument("submit: job id already in use");
+                for (size_t q = 0; q < i; ++q)
+                    if (jobs[q]->id == jobs[i]->id) throw std::invalid_argument("submit: job id already in use");
+            }
+            for (size_t i = 0; i < count; ++i) {
+                known_[jobs[i]->id] = jobs[i];
+                arrivals_.push_back(jobs[i]);
+            }
+            st_.jobs_submitted += count;
+            arrived_.fetch_add(count, std::memory_order_release);
         }
         cv_.notify_all();
+    }
+
+    void submit(uint64_t id, int member, const float* llr, int count, bool dev)
+    {
+        const auto j = make_job(id, member, llr, count, dev);
+        enqueue(&j, 1);
     }
 
     int poll(int max, unsigned long long* ids)
@@ -692,11 +713,21 @@ int sicc_server_drive(unsigned long long handle, int n_jobs, const int* member, 
             throw std::invalid_argument("server_drive: empty schedule");
         const uint64_t base_id = 0x8000000000000000ull + (now_us() << 20);
         const auto t0 = std::chrono::steady_clock::now();
-        for (int j = 0; j < n_jobs; ++j) {
-            const auto due = t0 + std::chrono::nanoseconds(static_cast<long long>(arrival_us[j] * 1000.0));
-            while (std::chrono::steady_clock::now() < due) {
+        auto due_at = [&](int j) { return t0 + std::chrono::nanoseconds(static_cast<long long>(arrival_us[j] * 1000.0)); };
+        // every job already due goes to the server in one group (one lock
+        // acquisition), as a client thread holding a backlog would submit it
+        std::vector<std::shared_ptr<Job>> group;
+        for (int j = 0; j < n_jobs;) {
+            while (std::chrono::steady_clock::now() < due_at(j)) {
             }
-            s->submit(base_id + j, member[j], reinterpret_cast<const float*>(d_llr_ptrs[j]), count[j], true);
+            const auto now = std::chrono::steady_clock::now();
+            group.clear();
+            while (j < n_jobs && group.size() < 256 && due_at(j) <= now) {
+                group.push_back(s->make_job(base_id + j, member[j], reinterpret_cast<const float*>(d_llr_ptrs[j]),
+                                            count[j], true));
+                ++j;
+            }
+            s->enqueue(group.data(), group.size());
         }
         s->drain();
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
