@@ -255,7 +255,7 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
                 unsigned hb[MAXDC], qb[MAXDC];
-                float m1 = FLT_MAX, m2 = FLT_MAX;
+                float av[MAXDC];
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
                     const float a = app[em[s][j] >> 16];
@@ -272,11 +272,30 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                     }
                     qv[s][j] = q;
                     qb[j] = __float_as_uint(q);
-                    const float av = fabsf(q);
-                    // m1 <= m2 always, so min(m2, max(m1, av)) is the median of the three
-                    m2 = __builtin_amdgcn_fmed3f(m1, m2, av);
-                    m1 = fminf(m1, av);
+                    av[j] = fabsf(q);
                 }
+                // smallest and second smallest |q| by a pairwise tree (depth log2 MAXDC
+                // instead of a MAXDC-long chain): pairs give (min, max); two (lo, second)
+                // pairs merge to (min lo, min(max lo, min second)).  Both are exact
+                // order statistics, capped at FLT_MAX as a sequential scan from
+                // (FLT_MAX, FLT_MAX) leaves them.
+                static_assert((MAXDC & (MAXDC - 1)) == 0, "row degree bound must be a power of two");
+                float lo[MAXDC / 2], se[MAXDC / 2];
+#pragma unroll
+                for (int i = 0; i < MAXDC / 2; ++i) {
+                    lo[i] = fminf(av[2 * i], av[2 * i + 1]);
+                    se[i] = fmaxf(av[2 * i], av[2 * i + 1]);
+                }
+#pragma unroll
+                for (int w = MAXDC / 2; w > 1; w >>= 1)
+#pragma unroll
+                    for (int i = 0; i < w / 2; ++i) {
+                        const float l = fminf(lo[2 * i], lo[2 * i + 1]);
+                        se[i] = fminf(fmaxf(lo[2 * i], lo[2 * i + 1]), fminf(se[2 * i], se[2 * i + 1]));
+                        lo[i] = l;
+                    }
+                const float m1 = fminf(lo[0], FLT_MAX);
+                const float m2 = fminf(se[0], FLT_MAX);
                 synw += __popcll(__ballot((xor_reduce(hb) >> 31) != 0u));
                 m1s[s] = m1;
                 m2s[s] = m2;
