@@ -27,7 +27,11 @@ extern "C" {
 #endif
 
 /* Members are decoder handles (stream BP-OSD or BP CSR decoders); the server
- * keeps them alive.  max_jobs_per_launch <= 0: unlimited. */
+ * keeps them alive.  max_jobs_per_launch <= 0: unlimited.  Besides the pool,
+ * each slot allocates its launch staging at creation, at its largest size:
+ * per member max_launch_codewords x (4n + k + 2) bytes of device memory (+ 4n
+ * for the OSD hand-off of BP-OSD members) and max_launch_codewords x
+ * (ceil(k_max / 8) + 2) bytes of pinned results. */
 int sicc_server_create(const unsigned long long* decoders, int count, unsigned long long budget_bytes, int slots,
                        int max_jobs_per_launch, int max_launch_codewords, unsigned long long* out_handle);
 int sicc_server_destroy(unsigned long long handle);
