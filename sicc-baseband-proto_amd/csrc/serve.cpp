@@ -112,6 +112,30 @@ public:
     Server(std::vector<std::shared_ptr<Decoder>> members, size_t budget, int slots, int max_jobs, int max_cw)
         : members_(std::move(members)), pool_(budget), max_jobs_(max_jobs), max_cw_(max_cw)
     {
+        try {
+            setup(slots, budget);
+        } catch (...) {
+            release();  // the destructor does not run for a constructor that throws
+            throw;
+        }
+        thread_ = std::thread([this] { loop(); });
+    }
+
+    ~Server()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        wake_.store(true, std::memory_order_release);
+        cv_.notify_all();
+        if (thread_.joinable()) thread_.join();
+        release();
+    }
+
+private:
+    void setup(int slots, size_t budget)
+    {
         SICC_HIP(hipGetDevice(&device_));
         SICC_HIP(hipMalloc(reinterpret_cast<void**>(&base_), budget));
         SICC_HIP(hipStreamCreateWithFlags(&admit_stream_, hipStreamNonBlocking));
@@ -142,18 +166,11 @@ public:
                 d.eng->reserve(osd >= 0 ? max_cw_ : 0, &d.bp, &s.scratch[m]);
             }
         }
-        thread_ = std::thread([this] { loop(); });
     }
 
-    ~Server()
+    // HIP handles and pinned buffers (device buffers free themselves).
+    void release()
     {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        wake_.store(true, std::memory_order_release);
-        cv_.notify_all();
-        if (thread_.joinable()) thread_.join();
         for (auto& s : slots_) {
             if (s.stream) (void)hipStreamSynchronize(s.stream);
             if (s.h_res) (void)hipHostFree(s.h_res);
@@ -167,7 +184,13 @@ public:
         }
         if (admit_event_) (void)hipEventDestroy(admit_event_);
         if (base_) (void)hipFree(base_);
+        slots_.clear();
+        admit_stream_ = nullptr;
+        admit_event_ = nullptr;
+        base_ = nullptr;
     }
+
+public:
 
     // Device memory a job holds from admission to completion: its LLR rows and
     // its share of the packed launch (LLR rows, results, OSD hand-off rows).

@@ -161,3 +161,22 @@ def test_ragged_info_width_round_trip():
                 assert np.array_equal(info[i], o["hard"][code.info_cols]), (j, i)
                 assert bool(ok[i]) == o["ok"] and not used[i], (j, i)
     dec.close()
+
+
+def test_failed_create_is_clean():
+    """A budget the device cannot hold fails server creation with an error status
+    (everything allocated before the failure is released), and the next server works."""
+    print("[case] server: create with an unallocatable budget", flush=True)
+    decs, codes = _members()
+    with pytest.raises(sa.SiccError):
+        sa.Server(decs[:1], budget_bytes=1 << 52, slots=2)
+    jobs = _jobs(codes[:1], 3, 8)
+    with sa.Server(decs[:1], budget_bytes=64 << 20, slots=2) as srv:
+        for j, (mem, llr) in enumerate(jobs):
+            srv.submit(j, mem, llr)
+        srv.drain()
+        for j, (mem, llr) in enumerate(jobs):
+            info, ok, used, _ = srv.fetch(j, llr.shape[0], codes[mem].k)
+            assert info.shape == (llr.shape[0], codes[mem].k)
+    for d in decs:
+        d.close()
