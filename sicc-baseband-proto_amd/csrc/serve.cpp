@@ -29,13 +29,16 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
-// First-fit allocator over [0, cap) with coalescing.
+// First-fit allocator over [0, cap) with coalescing, plus reservations that
+// count against the same capacity without taking address space (jobs whose
+// LLRs already live in the caller's device memory).
 class Pool {
 public:
     explicit Pool(size_t cap) : cap_(cap) { free_[0] = cap; }
     size_t alloc(size_t bytes)
     {
         bytes = align_up(bytes);
+        if (used_ + bytes > cap_) return SIZE_MAX;
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second >= bytes) {
                 const size_t off = it->first;
@@ -67,6 +70,15 @@ public:
             }
         }
     }
+    bool reserve(size_t bytes)
+    {
+        bytes = align_up(bytes);
+        if (used_ + bytes > cap_) return false;
+        used_ += bytes;
+        high_ = std::max(high_, used_);
+        return true;
+    }
+    void unreserve(size_t bytes) { used_ -= align_up(bytes); }
     size_t capacity() const { return cap_; }
     size_t high_water() const { return high_; }
 
@@ -90,7 +102,7 @@ struct Job {
     size_t grp_base = 0;           // this job's group block in the slot's pinned results
     int row = 0;                   // first row of the job inside its group
     int grp_cw = 0;                // rows in the group
-    std::vector<uint8_t> info, ok, used;  // info bit-packed: ceil(k/8) bytes per codeword
+    std::vector<uint8_t> res;  // [info bit-packed, ceil(k/8) bytes per codeword | ok | used]
 };
 
 struct Slot {
@@ -276,18 +288,19 @@ public:
             j = it->second;
             known_.erase(it);
         }
+        const int k = members_[j->member]->eng->code().k;
+        const size_t kb = (static_cast<size_t>(k) + 7) / 8;
+        const size_t cnt = static_cast<size_t>(j->count);
         if (info) {
             // unpack: bit i of a codeword sits at byte i/8, bit i%8
-            const int k = members_[j->member]->eng->code().k;
-            const size_t kb = (static_cast<size_t>(k) + 7) / 8;
-            for (int r = 0; r < j->count; ++r) {
-                const uint8_t* src = j->info.data() + r * kb;
-                uint8_t* dst = info + static_cast<size_t>(r) * k;
+            for (size_t r = 0; r < cnt; ++r) {
+                const uint8_t* src = j->res.data() + r * kb;
+                uint8_t* dst = info + r * k;
                 for (int i = 0; i < k; ++i) dst[i] = (src[i >> 3] >> (i & 7)) & 1u;
             }
         }
-        if (ok) std::memcpy(ok, j->ok.data(), j->ok.size());
-        if (used) std::memcpy(used, j->used.data(), j->used.size());
+        if (ok) std::memcpy(ok, j->res.data() + cnt * kb, cnt);
+        if (used) std::memcpy(used, j->res.data() + cnt * (kb + 1), cnt);
         if (lat) *lat = static_cast<double>(j->t_done - j->t_submit);
     }
 
@@ -430,16 +443,21 @@ private:
                 const size_t row = static_cast<size_t>(j->row);
                 const size_t gcw = static_cast<size_t>(j->grp_cw);
                 const uint8_t* g = s->h_res + j->grp_base;
-                j->info.assign(g + row * kb, g + (row + n) * kb);
-                j->ok.assign(g + gcw * kb + row, g + gcw * kb + row + n);
-                j->used.assign(g + gcw * (kb + 1) + row, g + gcw * (kb + 1) + row + n);
+                j->res.resize(n * (kb + 2));
+                uint8_t* o = j->res.data();
+                std::memcpy(o, g + row * kb, n * kb);
+                std::memcpy(o + n * kb, g + gcw * kb + row, n);
+                std::memcpy(o + n * (kb + 1), g + gcw * (kb + 1) + row, n);
             }
         }
         lk.lock();
         for (Slot* s : done_) {
             for (auto& j : s->jobs) {
                 j->t_done = t;
-                pool_.release(j->off, j->bytes);
+                if (j->off == SIZE_MAX)
+                    pool_.unreserve(j->bytes);
+                else
+                    pool_.release(j->off, j->bytes);
                 j->host_copy = std::vector<float>();
                 completed_.push_back(j->id);
                 ++st_.jobs_completed;
@@ -458,8 +476,11 @@ private:
         while (!arrivals_.empty()) {
             auto j = arrivals_.front();
             const size_t bytes = footprint(j->member, j->count);
-            const size_t off = pool_.alloc(bytes);
-            if (off == SIZE_MAX) break;  // in order: later jobs wait behind the head
+            // in order: later jobs wait behind the head.  A device-sourced job only
+            // reserves its share of the budget; a host job gets pool memory for its
+            // LLR copy.
+            const size_t off = j->src_dev ? SIZE_MAX : pool_.alloc(bytes);
+            if (j->src_dev ? !pool_.reserve(bytes) : off == SIZE_MAX) break;
             arrivals_.pop_front();
             j->off = off;
             j->bytes = bytes;
@@ -544,7 +565,9 @@ private:
     {
         {
             if (wait_admit) SICC_HIP(hipStreamWaitEvent(s.stream, admit_event_, 0));
-            std::vector<std::vector<Job*>> by(members_.size());
+            auto& by = by_;  // per member, reused across launches
+            by.resize(members_.size());
+            for (auto& v : by) v.clear();
             for (auto& j : s.jobs) by[j->member].push_back(j.get());
             size_t res = 0;
             int tbl = 0;
@@ -606,7 +629,8 @@ private:
     hipEvent_t admit_event_ = nullptr;
     bool admit_pending_ = false;
     std::vector<Slot> slots_;
-    std::vector<Slot*> done_;  // harvest's completed slots
+    std::vector<Slot*> done_;                 // harvest's completed slots
+    std::vector<std::vector<Job*>> by_;       // issue's per-member job lists
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
     std::deque<std::shared_ptr<Job>> arrivals_, resident_;
