@@ -14,6 +14,9 @@
  * flight: whenever a slot is free it packs every admitted, not yet launched
  * job (up to `max_launch_codewords`) into per-code groups and launches them, so
  * jobs that arrive while others run overlap with them on the device.
+ * While launches are in flight the scheduler thread polls their completion
+ * events (yielding between polls: one host core), so completions are seen
+ * within microseconds; an idle server sleeps on a condition variable.
  * slots = 1 and max_jobs_per_launch = 1 give the plain one-job-per-launch server.
  * Results are identical to decoding each job alone (per-codeword decoding is
  * independent of batch composition).
