@@ -262,7 +262,7 @@ struct PipelineCore {
         a.constellation = rx->d_constellation();
         a.precomp_h = have_precomp ? precomp_h.get() : nullptr;
         a.actual_h = have_actual ? actual_h.get() : nullptr;
-        a.twiddle = rx->d_twiddle();
+        a.twiddle = rx->d_twiddle_f();
         a.amplitude = cfg.tx_amplitude;
         a.noise_var = noise_var;
         a.noise_seed = seed;

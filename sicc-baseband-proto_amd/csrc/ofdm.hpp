@@ -99,7 +99,7 @@ struct OfdmTxArgs {
     const float2* constellation = nullptr;
     const float2* precomp_h = nullptr;    // optional [num_symbols][active_sc]
     const float2* actual_h = nullptr;     // optional [num_symbols][active_sc]
-    const double2* twiddle = nullptr;     // [nfft] exp(-2 pi i m / nfft)
+    const float2* twiddle = nullptr;      // [nfft] exp(-2 pi i m / nfft), single precision
     float amplitude = 0.2f;
     float noise_var = 0.0f;               // frequency-domain AWGN variance on data REs (0: none)
     unsigned long long noise_seed = 0;    // frame f of a batch uses noise_seed + f

@@ -175,6 +175,9 @@ void OfdmRx::init(const float2* preamble, const std::vector<int>& nat, const std
     up(w0_, w0.data(), w0.size());
     up(w1_, w1.data(), w1.size());
     up(twiddle_, tw.data(), tw.size());
+    std::vector<float2> twf(tw.size());
+    for (size_t m = 0; m < tw.size(); ++m) twf[m] = make_float2(static_cast<float>(tw[m].x), static_cast<float>(tw[m].y));
+    up(twiddle_f_, twf.data(), twf.size());
 }
 
 void OfdmRx::sync(const float2* d_rx, int frames, int span, OfdmSyncResult* d_out, int max_search, hipStream_t s) const

@@ -37,6 +37,7 @@ public:
     const float2* d_constellation() const { return constellation_.get(); }
     const int* d_used_natural() const { return used_natural_.get(); }
     const double2* d_twiddle() const { return twiddle_.get(); }
+    const float2* d_twiddle_f() const { return twiddle_f_.get(); }  // the transmitter's FP32 FFT
 
 private:
     void init(const float2* preamble, const std::vector<int>& nat, const std::vector<int>& ps,
@@ -51,6 +52,7 @@ private:
     DevBuf<int> pilot_symbols_, data_symbols_, used_natural_, p0_, p1_;
     DevBuf<float> w0_, w1_;
     DevBuf<double2> twiddle_;
+    DevBuf<float2> twiddle_f_;
     // LS channel estimates of the last demod batch; demods on one receiver
     // must be issued on one stream (or not overlap)
     mutable DevBuf<float2> h_est_;
