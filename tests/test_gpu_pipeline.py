@@ -1,9 +1,10 @@
 """GPU parity of usrp_gpu_pipeline_bridge::Pipeline (through its C layer).
 
 TX: the HIP transmitter against the oracle's restatement of the reference GPU
-transmitter (frame within 1e-5 of its peak: the inverse FFT is FP64 on both
-sides by different algorithms, and the AWGN goes through logf / cosf, whose
-last-ulp rounding differs between the device and host libraries).  RX: sync
+transmitter (frame within 1e-5 of its peak: the device inverse FFT is FP32,
+like the reference GPU transmitter's single-precision cuFFT, the oracle's FP64,
+and the AWGN goes through logf / cosf, whose last-ulp rounding differs between
+the device and host libraries).  RX: sync
 and LLRs against the oracle's find_frame / demod_frame, with the bound of the
 batched receiver tests.  Loopback: noise-free TX -> RX gives sign(LLR) = coded
 bit (the reference pipeline bench's known-answer check).
