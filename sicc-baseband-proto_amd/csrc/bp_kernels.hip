@@ -55,11 +55,11 @@ __global__ void zero_i32_kernel(int* p, int count)
 }
 
 // One workgroup per table entry; float4 moves when both ends are 16-byte aligned.
-__global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* table, float* dst, int row_len)
+__global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* table)
 {
     const GatherEntry e = table[blockIdx.x];
-    const size_t len = static_cast<size_t>(e.count) * row_len;
-    float* d = dst + static_cast<size_t>(e.dst_row) * row_len;
+    const size_t len = static_cast<size_t>(e.count) * e.row_len;
+    float* d = e.dst;
     const bool vec = ((reinterpret_cast<uintptr_t>(e.src) | reinterpret_cast<uintptr_t>(d)) & 15u) == 0 && (len & 3u) == 0;
     if (vec) {
         const float4* s4 = reinterpret_cast<const float4*>(e.src);
@@ -70,30 +70,43 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* tab
     }
 }
 
-// Result packing for the server's copy back: one wavefront per 64 information
-// bits of a row (ballot -> 8 bytes, bit i of a row at byte i/8, bit i%8), then
-// the trailing `flag_bytes` bytes (per-codeword flags) copied behind the packed
-// rows, one byte per thread in the blocks after the packing ones.
-__global__ __launch_bounds__(256) void pack_bits_kernel(const uint8_t* __restrict__ bits, int rows, int k,
-                                                        const uint8_t* __restrict__ flags, int flag_bytes,
-                                                        uint8_t* __restrict__ out, int pack_blocks)
+// Result packing for the server's copy back, all code groups of a launch in one
+// kernel: a block finds its group (few groups, block0 ascending), then one
+// wavefront packs 64 information bits of a row (ballot -> 8 bytes, bit i of a
+// row at byte i/8, bit i%8), or, in the group's trailing blocks, one thread
+// copies one byte of the per-codeword flags behind the packed rows.
+__global__ __launch_bounds__(256) void pack_groups_kernel(const PackGroup* __restrict__ groups, int count)
 {
-    const int kb = (k + 7) >> 3;
-    if (static_cast<int>(blockIdx.x) >= pack_blocks) {
-        const int i = (blockIdx.x - pack_blocks) * 256 + threadIdx.x;
-        if (i < flag_bytes) out[static_cast<size_t>(rows) * kb + i] = flags[i];
+    // the table may sit in host memory: fetch the group starts in one parallel
+    // read into LDS instead of a chain of dependent reads over the bus
+    constexpr int kLdsGroups = 64;
+    __shared__ int first[kLdsGroups];
+    int gi = 0;
+    if (count <= kLdsGroups) {
+        for (int i = threadIdx.x; i < count; i += blockDim.x) first[i] = groups[i].block0;
+        __syncthreads();
+        while (gi + 1 < count && static_cast<int>(blockIdx.x) >= first[gi + 1]) ++gi;
+    } else {
+        while (gi + 1 < count && static_cast<int>(blockIdx.x) >= groups[gi + 1].block0) ++gi;
+    }
+    const PackGroup g = groups[gi];
+    const int kb = (g.k + 7) >> 3;
+    const int b = static_cast<int>(blockIdx.x) - g.block0;
+    if (b >= g.pack_blocks) {
+        const int i = (b - g.pack_blocks) * 256 + threadIdx.x;
+        if (i < g.flag_bytes) g.out[static_cast<size_t>(g.rows) * kb + i] = g.flags[i];
         return;
     }
     const int lane = threadIdx.x & 63;
-    const int chunks = (k + 63) >> 6;
-    const long w = static_cast<long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (w >= static_cast<long>(rows) * chunks) return;  // whole wavefront
+    const int chunks = (g.k + 63) >> 6;
+    const long w = static_cast<long>(b) * 4 + (threadIdx.x >> 6);
+    if (w >= static_cast<long>(g.rows) * chunks) return;  // whole wavefront
     const long r = w / chunks;
     const int c = static_cast<int>(w - r * chunks);
     const int i = c * 64 + lane;
-    const unsigned long long m = __ballot(i < k && bits[r * k + i] != 0);
+    const unsigned long long m = __ballot(i < g.k && g.bits[r * g.k + i] != 0);
     const int nb = min(8, kb - c * 8);
-    if (lane < nb) out[r * kb + c * 8 + lane] = static_cast<uint8_t>(m >> (8 * lane));
+    if (lane < nb) g.out[r * kb + c * 8 + lane] = static_cast<uint8_t>(m >> (8 * lane));
 }
 
 // XOR of N words with three-input XORs (v_bitop3_b32, truth table 0x96):
@@ -603,26 +616,30 @@ void launch_zero_i32(int* p, int count, hipStream_t s)
     hipLaunchKernelGGL(zero_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, s, p, count);
 }
 
-void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int row_len, hipStream_t s)
+void launch_gather_rows(const GatherEntry* table, int entries, hipStream_t s)
 {
     if (entries <= 0) return;
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(entries), dim3(256), 0, s, table, dst, row_len);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(entries), dim3(256), 0, s, table);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("gather_rows_kernel launch: ") + hipGetErrorString(e));
 }
 
-void launch_pack_bits(const uint8_t* bits, int rows, int k, const uint8_t* flags, int flag_bytes, uint8_t* out,
-                      hipStream_t s)
+void launch_pack_groups(PackGroup* groups, int count, hipStream_t s)
 {
-    if (rows <= 0 || k <= 0 || flag_bytes < 0) throw std::invalid_argument("pack_bits: bad shape");
-    const long waves = static_cast<long>(rows) * ((k + 63) / 64);
-    const long pack_blocks = (waves + 3) / 4;
-    const long flag_blocks = (flag_bytes + 255) / 256;
-    if (pack_blocks + flag_blocks > 0x7fffffffL) throw std::invalid_argument("pack_bits: batch too large");
-    hipLaunchKernelGGL(pack_bits_kernel, dim3(static_cast<unsigned>(pack_blocks + flag_blocks)), dim3(256), 0, s, bits,
-                       rows, k, flags, flag_bytes, out, static_cast<int>(pack_blocks));
+    if (count <= 0) return;
+    long blocks = 0;
+    for (int i = 0; i < count; ++i) {
+        PackGroup& g = groups[i];
+        if (g.rows <= 0 || g.k <= 0 || g.flag_bytes < 0) throw std::invalid_argument("pack_groups: bad shape");
+        const long waves = static_cast<long>(g.rows) * ((g.k + 63) / 64);
+        g.block0 = static_cast<int>(blocks);
+        g.pack_blocks = static_cast<int>((waves + 3) / 4);
+        blocks += g.pack_blocks + (g.flag_bytes + 255) / 256;
+        if (blocks > 0x7fffffffL) throw std::invalid_argument("pack_groups: launch too large");
+    }
+    hipLaunchKernelGGL(pack_groups_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, groups, count);
     const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) throw std::runtime_error(std::string("pack_bits_kernel launch: ") + hipGetErrorString(e));
+    if (e != hipSuccess) throw std::runtime_error(std::string("pack_groups_kernel launch: ") + hipGetErrorString(e));
 }
 
 bool bp_team_supported(const BpCodeDev& c, int team)

@@ -86,18 +86,33 @@ struct OsdIo {
 };
 
 // Row gather for batch packing: entry i copies `count` rows of `row_len` floats
-// from `src` to rows [dst_row, dst_row + count) of the destination.  The table
-// may live in pinned host memory (read by the kernel over the bus).
+// from `src` to `dst`.  One launch serves every code of a mixed launch.  The
+// table may live in pinned host memory (read by the kernel over the bus).
 struct GatherEntry {
     const float* src;
+    float* dst;
     int count;
-    int dst_row;
+    int row_len;
 };
-void launch_gather_rows(const GatherEntry* table, int entries, float* dst, int row_len, hipStream_t s);
-// Bit-pack `rows` rows of k 0/1 bytes into ceil(k/8) bytes each (bit i at byte
-// i/8, bit i%8) and append `flag_bytes` bytes from `flags` behind them.
-void launch_pack_bits(const uint8_t* bits, int rows, int k, const uint8_t* flags, int flag_bytes, uint8_t* out,
-                      hipStream_t s);
+void launch_gather_rows(const GatherEntry* table, int entries, hipStream_t s);
+// Result packing of one code group: `rows` rows of k 0/1 bytes at `bits` become
+// ceil(k/8) bytes each at `out` (bit i at byte i/8, bit i%8), followed by
+// `flag_bytes` bytes copied from `flags`.  Blocks [block0, block0 + pack_blocks
+// + ceil(flag_bytes / 256)) of the launch serve the group (pack_blocks =
+// ceil(rows ceil(k/64) / 4)).
+struct PackGroup {
+    const uint8_t* bits;
+    const uint8_t* flags;
+    uint8_t* out;
+    int rows;
+    int k;
+    int flag_bytes;
+    int block0;
+    int pack_blocks;
+};
+// Fills block0 / pack_blocks of groups[0 .. count) and launches one kernel for
+// all of them; the table may live in pinned host memory.
+void launch_pack_groups(PackGroup* groups, int count, hipStream_t s);
 
 // Kernel launchers.  They validate shapes on the host and throw
 // std::invalid_argument when a code does not fit an instantiated variant.

@@ -116,6 +116,7 @@ struct Slot {
     uint8_t* h_res = nullptr;  // pinned results of one launch (bit-packed info), written by the GPU
     size_t h_res_bytes = 0;
     GatherEntry* table = nullptr;  // pinned gather table of one launch
+    PackGroup* groups = nullptr;   // pinned result-packing table, one group per member
     int table_cap = 0;
 };
 
@@ -166,6 +167,8 @@ private:
             s.table_cap = max_cw_;  // at most one entry per codeword
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.table), sizeof(GatherEntry) * s.table_cap,
                                    hipHostMallocDefault));
+            SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.groups), sizeof(PackGroup) * members_.size(),
+                                   hipHostMallocDefault));
             // Launch staging at its largest size up front: a buffer grown in the
             // serving loop would hipFree (a device-wide wait behind every launch in
             // flight) and stall the pipeline just when launches get large.
@@ -187,6 +190,7 @@ private:
             if (s.stream) (void)hipStreamSynchronize(s.stream);
             if (s.h_res) (void)hipHostFree(s.h_res);
             if (s.table) (void)hipHostFree(s.table);
+            if (s.groups) (void)hipHostFree(s.groups);
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.stream) (void)hipStreamDestroy(s.stream);
         }
@@ -569,52 +573,56 @@ private:
             by.resize(members_.size());
             for (auto& v : by) v.clear();
             for (auto& j : s.jobs) by[j->member].push_back(j.get());
-            size_t res = 0;
+            // rows of every job in its code group, and one gather launch for every
+            // group of more than one job (a single job is decoded in place)
             int tbl = 0;
             for (size_t m = 0; m < by.size(); ++m) {
                 if (by[m].empty()) continue;
-                Decoder& d = *members_[m];
-                const LdpcCode& c = d.eng->code();
-                const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
+                const int n = members_[m]->eng->code().n;
                 int cw = 0;
                 for (Job* j : by[m]) {
                     j->row = cw;
                     cw += j->count;
                 }
-                const float* llr = nullptr;
-                if (by[m].size() == 1) {
-                    llr = by[m][0]->d_llr;
-                } else {
+                if (by[m].size() > 1) {
                     // one gather workgroup per chunk of about 4096 LLRs, so a few
                     // large jobs still spread over the whole GPU
-                    const int ch = std::max(1, 4096 / c.n);
-                    GatherEntry* t = s.table + tbl;
-                    int ne = 0;
+                    const int ch = std::max(1, 4096 / n);
+                    float* dst = s.pack_llr[m].get();
                     for (Job* j : by[m])
                         for (int r = 0; r < j->count; r += ch)
-                            t[ne++] = {j->d_llr + static_cast<size_t>(r) * c.n, std::min(ch, j->count - r), j->row + r};
-                    launch_gather_rows(t, ne, s.pack_llr[m].get(), c.n, s.stream);
-                    tbl += ne;
-                    llr = s.pack_llr[m].get();
+                            s.table[tbl++] = {j->d_llr + static_cast<size_t>(r) * n,
+                                              dst + static_cast<size_t>(j->row + r) * n, std::min(ch, j->count - r), n};
                 }
+            }
+            launch_gather_rows(s.table, tbl, s.stream);
+            size_t res = 0;
+            int ng = 0;
+            for (size_t m = 0; m < by.size(); ++m) {
+                if (by[m].empty()) continue;
+                Decoder& d = *members_[m];
+                const LdpcCode& c = d.eng->code();
+                const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
+                const int cw = by[m].back()->row + by[m].back()->count;
+                const float* llr = by[m].size() == 1 ? by[m][0]->d_llr : s.pack_llr[m].get();
                 uint8_t* o = s.pack_out[m].get();
                 DecodeOut out;
                 out.info = o;
                 out.bp_ok = o + static_cast<size_t>(cw) * c.k;
                 out.used_osd = o + static_cast<size_t>(cw) * (c.k + 1);
                 d.eng->decode(llr, cw, d.bp, osd, d.osd, out, s.stream, &s.scratch[m]);
-                // bit-pack the information bits (8x less to move and to read on the
-                // host) with the ok / used flags behind them, written by the kernel
-                // straight into the slot's pinned results: no copy-engine transfer,
-                // whose D2H submissions stalled the host for milliseconds at times
-                const size_t bytes = static_cast<size_t>(cw) * ((c.k + 7) / 8 + 2);
-                launch_pack_bits(o, cw, c.k, out.bp_ok, 2 * cw, s.h_res + res, s.stream);
+                // results leave bit-packed (8x less to move and to read on the host),
+                // ok / used flags behind them, written by the packing kernel straight
+                // into the slot's pinned results: no copy-engine transfer, whose D2H
+                // submissions stalled the host for milliseconds at times
+                s.groups[ng++] = {o, out.bp_ok, s.h_res + res, cw, c.k, 2 * cw, 0, 0};
                 for (Job* j : by[m]) {
                     j->grp_base = res;
                     j->grp_cw = cw;
                 }
-                res += bytes;
+                res += static_cast<size_t>(cw) * ((c.k + 7) / 8 + 2);
             }
+            launch_pack_groups(s.groups, ng, s.stream);
             SICC_HIP(hipEventRecord(s.done, s.stream));
         }
     }
