@@ -180,3 +180,34 @@ def test_failed_create_is_clean():
             assert info.shape == (llr.shape[0], codes[mem].k)
     for d in decs:
         d.close()
+
+
+def test_many_code_groups_in_one_launch():
+    """70 member codes, one job each, released together: one launch carries 70
+    code groups (more than the packing kernel stages in LDS); every codeword
+    equals the oracle."""
+    print("[case] server: 70 code groups in one launch", flush=True)
+    rng = np.random.default_rng(21)
+    decs, codes, jobs = [], [], []
+    for i in range(70):
+        m, rp, ci = sa.qc_code(128, 4, 3, seed=500 + i)
+        decs.append(sa.Decoder.bp_csr(128, m, rp, ci, max_iter=20, alpha=0.75, schedule=0))
+        code = oracle.Code(128, m, rp, ci, True)
+        codes.append(code)
+        info = rng.integers(0, 2, (3, code.k), dtype=np.uint8)
+        jobs.append(oracle.awgn_llr(code.encode(info), 3.0, code.k / 128, rng))
+    with sa.Server(decs, budget_bytes=64 << 20, slots=1, max_launch_codewords=4096) as srv:
+        srv.hold(True)
+        for j, llr in enumerate(jobs):
+            srv.submit(j, j, llr)
+        srv.hold(False)
+        srv.drain()
+        assert srv.stats().launches == 1
+        for j, llr in enumerate(jobs):
+            info, ok, used, _ = srv.fetch(j, llr.shape[0], codes[j].k)
+            for i in range(llr.shape[0]):
+                o = codes[j].nms(llr[i], max_iter=20, alpha=0.75)
+                assert np.array_equal(info[i], o["hard"][codes[j].info_cols]), (j, i)
+                assert bool(ok[i]) == o["ok"], (j, i)
+    for d in decs:
+        d.close()
