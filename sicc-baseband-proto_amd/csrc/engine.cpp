@@ -203,8 +203,11 @@ const LongTables& Engine::long_tables(bool layered)
 BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool layered, DecodeScratch& sc)
 {
     BpLongScratch ls;
-    ls.blocks = cus_ * 2;
     ls.app_in_lds = bp_long_app_fits_lds(code_.n);
+    // codewords in flight: four workgroups per CU when the APP sits in LDS (the
+    // tail of a batch balances better; N16200 decodes ~5 % faster than with two),
+    // two when it lives in global memory (more would only add scratch traffic)
+    ls.blocks = cus_ * (ls.app_in_lds ? 4 : 2);
     const bool need_q = !layered && bp.damping != 0.0f;
     sc.ensure_long(ls.blocks, t.dev.slots, code_.n, need_q, !ls.app_in_lds);
     ls.r = sc.long_r.get();
