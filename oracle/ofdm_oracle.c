@@ -171,6 +171,41 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
                         int n_data, const int* data_sym, int bits_per_symbol, const float* constellation,
                         double cfo_hz, double fs, double noise_var, float* llr_out)
 {
+    oracle_demod_frame_ex(frame, nfft, cp, pre_len, num_symbols, active_sc, used_shifted, n_pilot, pilot_sym,
+                          pilot_grid, n_data, data_sym, bits_per_symbol, constellation, cfo_hz, fs, noise_var,
+                          llr_out, NULL, NULL);
+}
+
+/* LINK:2429-2441 nearest_constellation_symbol: the first point of least
+ * squared distance (strict < in table order). */
+static int nearest_point(float er, float ei, const float* constellation, int M)
+{
+    int best = 0;
+    float best_d2 = INFINITY;
+    for (int i = 0; i < M; ++i) {
+        const float d2 = normf(er - constellation[2 * i], ei - constellation[2 * i + 1]);
+        if (d2 < best_d2) {
+            best_d2 = d2;
+            best = i;
+        }
+    }
+    return best;
+}
+
+/* As oracle_demod_frame, plus the frame's SNR and EVM estimates of
+ * LINK:5431-5467 when snr_db / evm_rms are non-null: over every data resource
+ * element (y, h), h2 = max(|h|^2, 1e-9), eq0 = y conj(h) / h2, hard = the
+ * nearest constellation point to eq0; sums of |y - h hard|^2, |eq0 - hard|^2,
+ * |hard|^2 and |h|^2 in double in (data symbol, subcarrier) order; with the
+ * known noise variance nv = max(noise_var, 1e-8):
+ * snr_db = 10 log10(max(mean |h|^2 / nv, 1e-12)),
+ * evm_rms = sqrt(max(sum |eq0 - hard|^2, 0) / max(sum |hard|^2, 1e-12)). */
+void oracle_demod_frame_ex(const float* frame, int nfft, int cp, int pre_len, int num_symbols, int active_sc,
+                           const int* used_shifted, int n_pilot, const int* pilot_sym, const float* pilot_grid,
+                           int n_data, const int* data_sym, int bits_per_symbol, const float* constellation,
+                           double cfo_hz, double fs, double noise_var, float* llr_out, double* snr_db,
+                           double* evm_rms)
+{
     const int sym_len = nfft + cp;
     const int frame_len = pre_len + num_symbols * sym_len;
     float* y = (float*)malloc(sizeof(float) * 2 * (size_t)num_symbols * active_sc);
@@ -216,6 +251,7 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
     const int M = 1 << bits_per_symbol;
     if (noise_var < 1e-8) noise_var = 1e-8;
     const float nv = (float)noise_var;
+    double s_resid = 0.0, s_err = 0.0, s_ref = 0.0, s_h = 0.0;
     for (int di = 0; di < n_data; ++di) {
         const int sym = data_sym[di];
         int hi = 0;
@@ -236,6 +272,18 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
             const float yr = y[2 * ((size_t)sym * active_sc + sc)], yi = y[2 * ((size_t)sym * active_sc + sc) + 1];
             const float tr = yr * hr - yi * (-hi_);
             const float ti = yr * (-hi_) + yi * hr;
+            if (snr_db || evm_rms) {
+                /* LINK:5431-5439 */
+                const float qr = tr / h2, qi = ti / h2;
+                const int hp_i = nearest_point(qr, qi, constellation, M);
+                const float cr = constellation[2 * hp_i], ci = constellation[2 * hp_i + 1];
+                const float rr = yr - (hr * cr - hi_ * ci);
+                const float ri = yi - (hr * ci + hi_ * cr);
+                s_resid += (double)normf(rr, ri);
+                s_err += (double)normf(qr - cr, qi - ci);
+                s_ref += (double)normf(cr, ci);
+                s_h += (double)normf(hr, hi_);
+            }
             const float den = h2 + nv;
             const float er = tr / den, ei = ti / den;
             double s2 = noise_var / (double)h2;
@@ -255,6 +303,13 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
             }
         }
     }
+    (void)s_resid;  /* the decision-directed noise estimate: unused with a known variance */
+    if (snr_db) {
+        const double cnt = (double)((size_t)n_data * active_sc > 0 ? (size_t)n_data * active_sc : 1);
+        const double r = (s_h / cnt) / noise_var;
+        *snr_db = 10.0 * log10(r > 1e-12 ? r : 1e-12);
+    }
+    if (evm_rms) *evm_rms = sqrt((s_err > 0.0 ? s_err : 0.0) / (s_ref > 1e-12 ? s_ref : 1e-12));
     free(y);
     free(re);
     free(im);

@@ -202,8 +202,10 @@ def find_frame(buf: np.ndarray, half_len: int, preamble: np.ndarray, refine_radi
     return bool(ok), start.value, peak.value, cfo.value, coarse.value
 
 
-def demod_frame(frame: np.ndarray, phy, cfo_hz: float, noise_var: float) -> np.ndarray:
-    """LINK:5331-5521 demodulation of one frame (starting at the preamble) to LLRs."""
+def demod_frame(frame: np.ndarray, phy, cfo_hz: float, noise_var: float, stats: bool = False):
+    """LINK:5331-5521 demodulation of one frame (starting at the preamble) to LLRs;
+    with stats, also the frame's SNR (dB) and EVM estimates (LINK:5431-5467):
+    returns (llr, snr_db, evm_rms)."""
     iq = _iq(frame)
     used = np.ascontiguousarray(phy.used_shifted, np.int32)
     ps = np.ascontiguousarray(phy.pilot_symbols, np.int32)
@@ -211,11 +213,35 @@ def demod_frame(frame: np.ndarray, phy, cfo_hz: float, noise_var: float) -> np.n
     pg = _iq(phy.pilot_grid.reshape(-1))
     const = _iq(phy.constellation)
     out = np.zeros(len(ds) * phy.active_sc * phy.bits_per_symbol, np.float32)
-    load().oracle_demod_frame(_p(iq, f32p), phy.nfft, phy.cp, phy.pre_len, phy.num_symbols, phy.active_sc,
-                              _p(used, i32p), len(ps), _p(ps, i32p), _p(pg, f32p), len(ds), _p(ds, i32p),
-                              phy.bits_per_symbol, _p(const, f32p), C.c_double(cfo_hz), C.c_double(phy.fs),
-                              C.c_double(noise_var), _p(out, f32p))
+    snr = C.c_double(0.0)
+    evm = C.c_double(0.0)
+    load().oracle_demod_frame_ex(_p(iq, f32p), phy.nfft, phy.cp, phy.pre_len, phy.num_symbols, phy.active_sc,
+                                 _p(used, i32p), len(ps), _p(ps, i32p), _p(pg, f32p), len(ds), _p(ds, i32p),
+                                 phy.bits_per_symbol, _p(const, f32p), C.c_double(cfo_hz), C.c_double(phy.fs),
+                                 C.c_double(noise_var), _p(out, f32p), C.byref(snr) if stats else None,
+                                 C.byref(evm) if stats else None)
+    if stats:
+        return out, snr.value, evm.value
     return out
+
+
+def frame_finish(blocks: int, n: int, k: int, block_parity, block_iters, info, llr, ref_info=None,
+                 ref_coded=None, gated: bool = False) -> dict:
+    """LINK:5482-5591 per-frame aggregation of one frame's blocks (FrameDecodeResult
+    fields other than detection, peak, CFO, SNR and EVM)."""
+    bp = np.ascontiguousarray(block_parity, np.uint8)
+    bi = np.ascontiguousarray(block_iters, np.int32)
+    inf = np.ascontiguousarray(info, np.uint8).reshape(-1)
+    ll = np.ascontiguousarray(llr, np.float32).reshape(-1)
+    ri = None if ref_info is None else np.ascontiguousarray(ref_info, np.uint8).reshape(-1)
+    rc = None if ref_coded is None else np.ascontiguousarray(ref_coded, np.uint8).reshape(-1)
+    po, it, be, pe, fo = (C.c_int(0) for _ in range(5))
+    fid = C.c_uint32(0)
+    load().oracle_frame_finish(blocks, n, k, _p(bp, u8p), _p(bi, i32p), _p(inf, u8p), _p(ri, u8p), _p(rc, u8p),
+                               _p(ll, f32p), int(gated), C.byref(po), C.byref(it), C.byref(fid), C.byref(be),
+                               C.byref(pe), C.byref(fo))
+    return {"parity_ok": po.value, "iterations": it.value, "frame_id": fid.value, "bit_errors": be.value,
+            "pre_fec_bit_errors": pe.value, "frame_ok": fo.value}
 
 
 def symbol_index_tables(num_symbols: int, pilot_symbols, data_symbols):

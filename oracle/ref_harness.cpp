@@ -219,4 +219,50 @@ int ref_decode_frame(const float* frame_iq, double cfo_hz, double known_noise_va
     }
 }
 
+// build_info_bits (LINK:4048-4056): the reference payload of a frame id.
+int ref_build_info_bits(unsigned frame_id, uint8_t* out)
+{
+    if (!g_have_phy || !g_have_code) return -1;
+    const auto b = build_info_bits(g_phy, g_code, frame_id);
+    std::memcpy(out, b.data(), b.size());
+    return static_cast<int>(b.size());
+}
+
+// decode_frame (LINK:5331) with the CPU min-sum FEC, every FrameDecodeResult
+// field: ires = {detected, parity_ok, frame_ok, frame_id, bit_errors,
+// pre_fec_bit_errors, iterations}, dres = {peak, cfo_hz, snr_db, evm_rms}.
+// expected_id < 0: the reference id is the decoded one.
+int ref_decode_frame_full(const float* frame_iq, double cfo_hz, double peak, double known_noise_var, int max_iter,
+                          double norm, double snr_gate_db, int reference_test, long long expected_id,
+                          uint8_t* info_out, long long* ires, double* dres)
+{
+    if (!g_have_phy || !g_have_code) return -1;
+    try {
+        std::vector<cf32> f(static_cast<size_t>(g_phy.frame_len()));
+        for (size_t i = 0; i < f.size(); ++i) f[i] = cf32(frame_iq[2 * i], frame_iq[2 * i + 1]);
+        std::optional<double> nv;
+        if (known_noise_var > 0.0) nv = known_noise_var;
+        std::optional<uint32_t> eid;
+        if (expected_id >= 0) eid = static_cast<uint32_t>(expected_id);
+        const FrameDecodeResult r = decode_frame(f, g_phy, g_code, nullptr, nullptr, cfo_hz, peak, max_iter, norm,
+                                                 snr_gate_db, reference_test != 0, nullptr, eid, nv);
+        if (!r.info_bits.empty()) std::memcpy(info_out, r.info_bits.data(), r.info_bits.size());
+        ires[0] = r.detected;
+        ires[1] = r.parity_ok;
+        ires[2] = r.frame_ok;
+        ires[3] = r.frame_id;
+        ires[4] = r.bit_errors;
+        ires[5] = r.pre_fec_bit_errors;
+        ires[6] = r.iterations;
+        dres[0] = r.peak;
+        dres[1] = r.cfo_hz;
+        dres[2] = r.snr_db;
+        dres[3] = r.evm_rms;
+        return static_cast<int>(r.info_bits.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 }  // extern "C"

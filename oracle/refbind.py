@@ -131,3 +131,30 @@ class RefPhy:
         if r < 0:
             raise RuntimeError(load().ref_last_error().decode())
         return info[:r], bool(ok.value)
+
+    def build_info_bits(self, frame_id: int, info_len: int) -> np.ndarray:
+        """build_info_bits (LINK:4048): the reference payload of a frame id."""
+        out = np.zeros(info_len, np.uint8)
+        r = load().ref_build_info_bits(C.c_uint(frame_id), _p(out, u8p))
+        if r < 0:
+            raise RuntimeError(load().ref_last_error().decode())
+        return out[:r]
+
+    def decode_frame_full(self, frame: np.ndarray, cfo_hz: float, peak: float, noise_var: float, max_iter: int,
+                          norm: float, info_len: int, snr_gate_db: float = -120.0, reference_test: bool = True,
+                          expected_id: int = -1) -> dict:
+        """decode_frame (LINK:5331-5603) with every FrameDecodeResult field."""
+        iq = np.ascontiguousarray(np.asarray(frame, np.complex64).view(np.float32))
+        info = np.zeros(info_len, np.uint8)
+        ires = np.zeros(7, np.int64)
+        dres = np.zeros(4, np.float64)
+        r = load().ref_decode_frame_full(_p(iq, f32p), C.c_double(cfo_hz), C.c_double(peak),
+                                         C.c_double(noise_var), max_iter, C.c_double(norm), C.c_double(snr_gate_db),
+                                         int(reference_test), C.c_longlong(expected_id), _p(info, u8p),
+                                         ires.ctypes.data_as(C.POINTER(C.c_longlong)), _p(dres, f64p))
+        if r < 0:
+            raise RuntimeError(load().ref_last_error().decode())
+        keys = ("detected", "parity_ok", "frame_ok", "frame_id", "bit_errors", "pre_fec_bit_errors", "iterations")
+        out = {k: int(v) for k, v in zip(keys, ires)}
+        out.update({"peak": dres[0], "cfo_hz": dres[1], "snr_db": dres[2], "evm_rms": dres[3], "info": info[:r]})
+        return out

@@ -23,6 +23,11 @@
  *                           argmin over explicit TEP set, basis independence, ties.
  *   oracle_ofdm_*           pinned: numpy FFT, closed-form QPSK LLR, brute force
  *                           sliding metric, noise-free known-answer loopback.
+ *   oracle_demod_frame_ex   pinned: SNR / EVM equal to the reference decode_frame
+ *                           (golden fixture written by its own CPU path).
+ *   oracle_frame_finish     pinned: every FrameDecodeResult field equal to the
+ *                           reference decode_frame in its reference-test mode
+ *                           (golden fixture); SNR-gate branch by construction.
  */
 #ifndef SICC_ORACLE_H
 #define SICC_ORACLE_H
@@ -83,6 +88,22 @@ void oracle_demod_frame(const float* frame, int nfft, int cp, int pre_len, int n
                         const float* pilot_grid, int n_data, const int* data_sym,
                         int bits_per_symbol, const float* constellation, double cfo_hz, double fs,
                         double noise_var, float* llr_out);
+/* oracle_demod_frame plus the SNR / EVM estimates of LINK:5431-5467 (known noise
+ * variance); snr_db / evm_rms may be NULL. */
+void oracle_demod_frame_ex(const float* frame, int nfft, int cp, int pre_len, int num_symbols,
+                           int active_sc, const int* used_shifted, int n_pilot, const int* pilot_sym,
+                           const float* pilot_grid, int n_data, const int* data_sym,
+                           int bits_per_symbol, const float* constellation, double cfo_hz, double fs,
+                           double noise_var, float* llr_out, double* snr_db, double* evm_rms);
+
+/* ---- per-frame FEC aggregation (LINK:5482-5591) ----
+ * block_parity / block_iters [blocks]; info [blocks*k] decoded; ref_info
+ * [blocks*k] and ref_coded [blocks*n] (its encoding) or NULL for no reference
+ * test; llr [blocks*n] (hard decisions for the pre-FEC count). */
+void oracle_frame_finish(int blocks, int n, int k, const uint8_t* block_parity, const int* block_iters,
+                         const uint8_t* info, const uint8_t* ref_info, const uint8_t* ref_coded,
+                         const float* llr, int gated, int* parity_ok, int* iterations, uint32_t* frame_id,
+                         int* bit_errors, int* pre_fec_bit_errors, int* frame_ok);
 
 /* ---- OFDM transmit, GPU transmitter semantics (PIPE:129-342) ----
  * info [blocks][k] bytes; gen_kn [k][n] bytes (generator path) or, when

@@ -61,6 +61,13 @@ struct OfdmDemodArgs {
     float* llr = nullptr;  // [frames][llr_per_frame]
     int llr_per_frame = 0;
     float2* h_est = nullptr;  // scratch [frames][n_pilot][active_sc] LS estimates
+    // Optional per-(frame, data symbol) sums over the symbol's active
+    // subcarriers, pad resource elements included, of |h|^2, |y - h hard|^2,
+    // |eq0 - hard|^2 and |hard|^2 (eq0 = y conj(h) / max(|h|^2, 1e-9), hard =
+    // the nearest constellation point): [frames][n_data][4] doubles, zeros for a
+    // frame sync did not deliver.  The SNR / EVM estimates of decode_frame
+    // (uhd_ldpc_ofdm_link.cpp:5431-5467) are ratios of their per-frame totals.
+    double* stats = nullptr;
 };
 
 void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s);

@@ -199,7 +199,7 @@ void OfdmRx::sync(const float2* d_rx, int frames, int span, OfdmSyncResult* d_ou
 }
 
 void OfdmRx::demod(const float2* d_rx, int frames, int span, const OfdmSyncResult* d_sync, float* d_llr,
-                   hipStream_t s) const
+                   hipStream_t s, double* d_stats) const
 {
     if (!d_rx || !d_sync || !d_llr || frames <= 0) throw std::invalid_argument("demod: empty batch");
     if (span < frame_samples()) throw std::invalid_argument("demod: rows shorter than one frame");
@@ -232,6 +232,7 @@ void OfdmRx::demod(const float2* d_rx, int frames, int span, const OfdmSyncResul
     a.use_pilots = cfg_.use_pilot_channel_est;
     a.llr = d_llr;
     a.llr_per_frame = llr_per_frame();
+    a.stats = d_stats;
     if (cfg_.use_pilot_channel_est) {
         const size_t need = static_cast<size_t>(frames) * n_pilot_ * cfg_.active_sc;
         if (h_est_.size() < need) {
@@ -248,7 +249,7 @@ namespace {
 struct OfdmHandle {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::unique_ptr<OfdmRx> rx;
+    std::shared_ptr<OfdmRx> rx;
     DevBuf<float2> h_rx;
     DevBuf<OfdmSyncResult> h_sync;
     DevBuf<float> h_llr;
@@ -297,6 +298,8 @@ std::shared_ptr<OfdmHandle> olookup(unsigned long long h)
 static_assert(sizeof(sicc_ofdm_sync_t) == sizeof(OfdmSyncResult), "sync result layout");
 
 }  // namespace
+
+std::shared_ptr<OfdmRx> ofdm_receiver(unsigned long long handle) { return olookup(handle)->rx; }
 }  // namespace sicc
 
 using namespace sicc;

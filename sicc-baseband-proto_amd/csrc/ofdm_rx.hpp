@@ -5,6 +5,7 @@
 #include "ofdm.hpp"
 #include "sicc_common.hpp"
 
+#include <memory>
 #include <vector>
 
 namespace sicc {
@@ -28,8 +29,11 @@ public:
     int llr_per_frame() const { return cfg_.blocks_per_frame * cfg_.code_n; }
 
     void sync(const float2* d_rx, int frames, int span, OfdmSyncResult* d_out, int max_search, hipStream_t s) const;
+    // d_stats (optional): [frames][n_data][4] per-symbol SNR / EVM sums (OfdmDemodArgs::stats).
     void demod(const float2* d_rx, int frames, int span, const OfdmSyncResult* d_sync, float* d_llr,
-               hipStream_t s) const;
+               hipStream_t s, double* d_stats = nullptr) const;
+    int n_data() const { return n_data_; }
+    int n_pilot() const { return n_pilot_; }
 
     // device tables shared with a transmitter of the same numerology
     const float2* d_preamble() const { return preamble_.get(); }
@@ -57,6 +61,10 @@ private:
     // must be issued on one stream (or not overlap)
     mutable DevBuf<float2> h_est_;
 };
+
+// The receiver behind a sicc_ofdm_* handle (kept alive by the returned
+// pointer); throws std::invalid_argument for an unknown handle.
+std::shared_ptr<OfdmRx> ofdm_receiver(unsigned long long handle);
 
 // Gray QAM constellation indexed by the symbol's bits MSB first, unit average
 // power (configure_modulation, uhd_ldpc_ofdm_link.cpp:2362-2414).

@@ -332,26 +332,46 @@ __global__ __launch_bounds__(kThreads) void ofdm_pilot_kernel(OfdmDemodArgs a)
     }
 }
 
+// Sum of four doubles over the workgroup; the total is valid in thread 0.
+__device__ __forceinline__ void block_sum4(double (&v)[4], double* red)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_xor(v[j], o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[4 * w + j] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < static_cast<int>(blockDim.x >> 6); ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] += red[4 * i + j];
+}
+
 // One workgroup per (frame, data symbol): FFT, interpolated channel, equalize,
 // max-log LLRs of the symbol's resource elements.
 template <int B>
 __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
 {
     extern __shared__ double2 sh[];
+    __shared__ double red4[4 * (kThreads / 64)];
     Cd* buf = reinterpret_cast<Cd*>(sh);
     const int f = blockIdx.x / a.n_data;
     const int di = blockIdx.x - f * a.n_data;
     const int A = a.active_sc;
     const OfdmSyncResult sr = a.sync[f];
     float* out = a.llr + static_cast<size_t>(f) * a.llr_per_frame;
+    double* st = a.stats ? a.stats + (static_cast<size_t>(f) * a.n_data + di) * 4 : nullptr;
     if (!frame_usable(a, sr)) {
         // no frame detected, or the frame runs past the row: zero LLRs (erasures)
         const int lo = di * A * B;
         const int hi = min(lo + A * B, a.llr_per_frame);
         for (int i = lo + static_cast<int>(threadIdx.x); i < hi; i += blockDim.x) out[i] = 0.0f;
+        if (st && threadIdx.x < 4) st[threadIdx.x] = 0.0;
         return;
     }
-    if (di * A * B >= a.llr_per_frame) return;  // a symbol of pad resource elements only
+    if (!st && di * A * B >= a.llr_per_frame) return;  // a symbol of pad resource elements only
     const float2* x = a.rx + static_cast<size_t>(f) * a.span + sr.start;
     symbol_fft(a, x, sr.cfo_hz_d, a.data_symbols[di], buf);
     constexpr int AX = B == 1 ? 1 : B / 2;  // leading (in-phase) bits of a symbol index
@@ -362,21 +382,47 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
     const float w0 = a.interp_w0[di], w1 = a.interp_w1[di];
     const float2* h0p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p0) * A : nullptr;
     const float2* h1p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p1) * A : nullptr;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // |h|^2, |y - h hard|^2, |eq0 - hard|^2, |hard|^2
     for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
         const int obit = (di * A + sc) * B;
-        if (obit >= a.llr_per_frame) continue;  // pad resource elements
+        if (!st && obit >= a.llr_per_frame) continue;  // pad resource elements
         float hr = 1.0f, hi = 0.0f;
         if (a.use_pilots) {
             const float2 h0 = h0p[sc], h1 = h1p[sc];
             hr = h0.x * w0 + h1.x * w1;
             hi = h0.y * w0 + h1.y * w1;
         }
-        float h2 = hr * hr + hi * hi;
-        if (h2 < 1e-9f) h2 = 1e-9f;
+        const float hn = hr * hr + hi * hi;
+        const float h2 = hn < 1e-9f ? 1e-9f : hn;
         const Cd yv = buf[a.used_natural[sc]];
         const float yr = static_cast<float>(yv.x), yi = static_cast<float>(yv.y);
         const float tr = yr * hr - yi * (-hi);
         const float ti = yr * (-hi) + yi * hr;
+        if (st) {
+            // decode_frame's SNR / EVM terms (uhd_ldpc_ofdm_link.cpp:5431-5439, 5457-5459):
+            // the nearest point to eq0 is the first of least squared distance
+            const float qr = tr / h2, qi = ti / h2;
+            float bd = INFINITY;
+            float cr = 0.0f, ci = 0.0f;
+            for (int i = 0; i < (1 << B); ++i) {
+                const float2 c = a.constellation[i];
+                const float dx = qr - c.x, dy = qi - c.y;
+                const float d2 = dx * dx + dy * dy;
+                if (d2 < bd) {
+                    bd = d2;
+                    cr = c.x;
+                    ci = c.y;
+                }
+            }
+            const float rr = yr - (hr * cr - hi * ci);
+            const float ri = yi - (hr * ci + hi * cr);
+            const float er0 = qr - cr, ei0 = qi - ci;
+            acc[0] += static_cast<double>(hn);
+            acc[1] += static_cast<double>(rr * rr + ri * ri);
+            acc[2] += static_cast<double>(er0 * er0 + ei0 * ei0);
+            acc[3] += static_cast<double>(cr * cr + ci * ci);
+            if (obit >= a.llr_per_frame) continue;  // pad resource elements carry no LLR
+        }
         const float den = h2 + nv;
         const float er = tr / den, ei = ti / den;
         double s2 = static_cast<double>(a.noise_var_d) / static_cast<double>(h2);
@@ -434,6 +480,12 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
             }
             if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) * inv_s2);
         }
+    }
+    if (st) {
+        block_sum4(acc, red4);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) st[j] = acc[j];
     }
 }
 

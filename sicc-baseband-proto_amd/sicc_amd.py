@@ -521,6 +521,98 @@ class OfdmReceiver:
                                                 C.c_void_p(d_llr), stream))
 
 
+class FrameConfig(C.Structure):
+    _fields_ = [("blocks_per_frame", C.c_int), ("snr_gate_db", C.c_double)]
+
+
+FRAME_RESULT_DTYPE = np.dtype([
+    ("detected", np.int32), ("parity_ok", np.int32), ("frame_ok", np.int32), ("frame_id", np.uint32),
+    ("bit_errors", np.int32), ("pre_fec_bit_errors", np.int32), ("iterations", np.int32), ("blocks", np.int32),
+    ("bp_ok_blocks", np.int32), ("osd_blocks", np.int32), ("snr_gated", np.int32), ("reserved", np.int32),
+    ("peak", np.float64), ("cfo_hz", np.float64), ("snr_db", np.float64), ("evm_rms", np.float64)])
+assert FRAME_RESULT_DTYPE.itemsize == 80
+
+
+class FrameDecoder:
+    """Owns one frame decoder (sicc_frame_* C ABI): an OFDM receiver (or none,
+    for LLR input) bound to an LDPC decoder, returning FrameDecodeResult records."""
+
+    def __init__(self, receiver: "OfdmReceiver | None", decoder: "Decoder", blocks_per_frame: int = 0,
+                 snr_gate_db: float = -1e30):
+        self.lib = load()
+        lib = self.lib
+        vp = C.c_void_p
+        lib.sicc_frame_decoder_create.argtypes = [C.c_ulonglong, C.c_ulonglong, C.POINTER(FrameConfig),
+                                                  C.POINTER(C.c_ulonglong)]
+        lib.sicc_frame_decoder_destroy.argtypes = [C.c_ulonglong]
+        lib.sicc_frame_decoder_reserve.argtypes = [C.c_ulonglong, C.c_int]
+        lib.sicc_frame_decoder_info.argtypes = [C.c_ulonglong, i32p, i32p, i32p, i32p]
+        lib.sicc_frame_receive_decode_device.argtypes = [C.c_ulonglong, vp, C.c_int, C.c_int] + [vp] * 8
+        lib.sicc_frame_demod_decode_device.argtypes = [C.c_ulonglong, vp, C.c_int, C.c_int] + [vp] * 8
+        lib.sicc_frame_decode_llr_device.argtypes = [C.c_ulonglong, vp, C.c_int] + [vp] * 6
+        lib.sicc_frame_receive_decode_host.argtypes = [C.c_ulonglong, vp, C.c_int, C.c_int, vp, vp, vp]
+        lib.sicc_frame_decode_llr_host.argtypes = [C.c_ulonglong, vp, C.c_int, vp, vp, vp]
+        cfg = FrameConfig(blocks_per_frame=blocks_per_frame, snr_gate_db=snr_gate_db)
+        h = C.c_ulonglong(0)
+        check(lib.sicc_frame_decoder_create(receiver.handle if receiver else 0, decoder.handle, C.byref(cfg),
+                                            C.byref(h)))
+        self.handle = h.value
+        b, n, k, ib = (C.c_int() for _ in range(4))
+        check(lib.sicc_frame_decoder_info(self.handle, C.byref(b), C.byref(n), C.byref(k), C.byref(ib)))
+        self.blocks, self.n, self.k, self.info_bits = b.value, n.value, k.value, ib.value
+
+    def close(self):
+        if self.handle:
+            check(self.lib.sicc_frame_decoder_destroy(self.handle))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def reserve(self, frames: int):
+        check(self.lib.sicc_frame_decoder_reserve(self.handle, frames))
+
+    def receive_decode_device(self, d_rx, frames, span, d_info, d_results, d_ref=None, d_sync=None, d_llr=None,
+                              d_block_ok=None, d_block_it=None, stream=None):
+        check(self.lib.sicc_frame_receive_decode_device(self.handle, d_rx, frames, span, d_ref, d_info, d_results,
+                                                        d_sync, d_llr, d_block_ok, d_block_it, stream))
+
+    def demod_decode_device(self, d_rx, frames, span, d_sync, d_info, d_results, d_ref=None, d_llr=None,
+                            d_block_ok=None, d_block_it=None, stream=None):
+        check(self.lib.sicc_frame_demod_decode_device(self.handle, d_rx, frames, span, d_sync, d_ref, d_info,
+                                                      d_results, d_llr, d_block_ok, d_block_it, stream))
+
+    def decode_llr_device(self, d_llr, frames, d_info, d_results, d_ref=None, d_block_ok=None, d_block_it=None,
+                          stream=None):
+        check(self.lib.sicc_frame_decode_llr_device(self.handle, d_llr, frames, d_ref, d_info, d_results,
+                                                    d_block_ok, d_block_it, stream))
+
+    def receive_decode_host(self, rx: np.ndarray, ref_info: np.ndarray | None = None):
+        rx = np.ascontiguousarray(np.asarray(rx, np.complex64))
+        frames, span = rx.shape
+        info = np.zeros((frames, self.info_bits), np.uint8)
+        res = np.zeros(frames, FRAME_RESULT_DTYPE)
+        ref = None if ref_info is None else np.ascontiguousarray(ref_info, np.uint8)
+        check(self.lib.sicc_frame_receive_decode_host(self.handle, rx.ctypes.data, frames, span,
+                                                      None if ref is None else ref.ctypes.data, info.ctypes.data,
+                                                      res.ctypes.data))
+        return info, res
+
+    def decode_llr_host(self, llr: np.ndarray, ref_info: np.ndarray | None = None):
+        llr = np.ascontiguousarray(llr, np.float32)
+        frames = llr.shape[0]
+        info = np.zeros((frames, self.info_bits), np.uint8)
+        res = np.zeros(frames, FRAME_RESULT_DTYPE)
+        ref = None if ref_info is None else np.ascontiguousarray(ref_info, np.uint8)
+        check(self.lib.sicc_frame_decode_llr_host(self.handle, llr.ctypes.data, frames,
+                                                  None if ref is None else ref.ctypes.data, info.ctypes.data,
+                                                  res.ctypes.data))
+        return info, res
+
+
 def find_frame(signal: np.ndarray, preamble: np.ndarray, half_len: int, threshold: float, fs: float):
     """cuda_osd_find_frame on the GPU -> (start_1based, peak, cfo_hz)."""
     lib = load()

@@ -10,7 +10,11 @@ functions return for them:
                      6 symbols, 128 subcarriers, 16-QAM, pilot every 3rd symbol): its
                      tables, build_tx_frame_from_info (LINK:4590), find_frame (LINK:5123),
                      fft_to_shifted_grid (LINK:3845), symbol_to_llr_maxlog (LINK:2552)
-                     and decode_frame (LINK:5331).
+                     and decode_frame (LINK:5331);
+  frame_small.npz    decode_frame in its reference-test mode (every FrameDecodeResult
+                     field, LINK:273-286) on 16 synchronized frames of that PHY, with
+                     the reference payloads (build_info_bits LINK:4048), with and
+                     without the SNR gate.
 Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -89,9 +93,73 @@ def ofdm():
     np.savez_compressed(os.path.join(HERE, "ofdm_small.npz"), **out)
 
 
+FRAME_SD = 0.0065  # time-domain noise std per component of the frame fixture
+
+
+def frames():
+    """frame_small.npz: decode_frame (LINK:5331-5603) in its reference-test mode on
+    16 frames of the small PHY, each through its own 3-tap channel, CFO and
+    offset at one noise level (a mix of decoded, failed and bit-error frames):
+    buffers, the reference sync (find_frame), the reference payloads
+    (build_info_bits LINK:4048) and every FrameDecodeResult field, without and
+    with an SNR gate."""
+    code = refbind.RefCode(ALIST, True)
+    phy = refbind.RefPhy(nfft=256, cp=18, num_symbols=6, active_sc=128, pre_half_len=128, modulation="16qam",
+                         pilot_period=3, rate=15.36e6)
+    rng = np.random.default_rng(20260103)
+    blocks = phy.coded_bits // code.n
+    ninfo = blocks * code.k
+    F, span = 16, phy.frame_len + 160
+    nv = 2 * FRAME_SD ** 2 * phy.nfft
+    gate = 14.0
+    bufs = np.zeros((F, span), np.complex64)
+    sync = np.zeros((F, 3), np.float64)  # start, peak, cfo
+    ids = np.zeros(F, np.int64)
+    ref_info = np.zeros((F, ninfo), np.uint8)
+    info = np.zeros((F, ninfo), np.uint8)
+    res_i = np.zeros((F, 7), np.int64)
+    res_d = np.zeros((F, 4), np.float64)
+    gate_i = np.zeros((F, 7), np.int64)
+    gate_d = np.zeros((F, 4), np.float64)
+    for f in range(F):
+        fid = int(rng.integers(0, 2 ** 31))
+        ri = phy.build_info_bits(fid, ninfo)
+        fr = phy.build_tx_frame(ri)
+        off = int(rng.integers(10, 100))
+        cfo = rng.uniform(-0.3, 0.3) * phy.fs / phy.nfft
+        taps = [1.0, 0.3 * np.exp(2j * np.pi * rng.uniform()), 0.1 * np.exp(2j * np.pi * rng.uniform())]
+        buf = np.zeros(span, np.complex128)
+        buf[off:off + phy.frame_len] = np.convolve(fr, taps)[: phy.frame_len]
+        buf *= np.exp(2j * np.pi * cfo * np.arange(span) / phy.fs)
+        buf += FRAME_SD * (rng.standard_normal(span) + 1j * rng.standard_normal(span))
+        buf = buf.astype(np.complex64)
+        ok, s, peak, c = phy.find_frame(buf)
+        assert ok, f
+        rx = buf[s:s + phy.frame_len]
+        r = phy.decode_frame_full(rx, c, peak, nv, 20, 0.75, ninfo, reference_test=True, expected_id=fid)
+        g = phy.decode_frame_full(rx, c, peak, nv, 20, 0.75, ninfo, snr_gate_db=gate, reference_test=True,
+                                  expected_id=fid)
+        keys = ("detected", "parity_ok", "frame_ok", "frame_id", "bit_errors", "pre_fec_bit_errors", "iterations")
+        bufs[f], sync[f], ids[f], ref_info[f], info[f] = buf, (s, peak, c), fid, ri, r["info"]
+        res_i[f] = [r[k] for k in keys]
+        res_d[f] = [r["peak"], r["cfo_hz"], r["snr_db"], r["evm_rms"]]
+        gate_i[f] = [g[k] for k in keys]
+        gate_d[f] = [g["peak"], g["cfo_hz"], g["snr_db"], g["evm_rms"]]
+    np.savez_compressed(
+        os.path.join(HERE, "frame_small.npz"),
+        dims=np.array([phy.nfft, phy.cp, phy.num_symbols, phy.active_sc, phy.pre_half_len, phy.bits_per_symbol,
+                       phy.frame_len, blocks]),
+        fs=np.array([phy.fs]), preamble=phy.preamble, used_shifted=phy.used_shifted,
+        pilot_symbols=phy.pilot_symbols, data_symbols=phy.data_symbols, pilot_grid=phy.pilot_grid,
+        constellation=phy.constellation, noise_var=np.array([nv]), bp=np.array([20, 0.75]), bufs=bufs, sync=sync,
+        frame_ids=ids, ref_info=ref_info, info=info, res_int=res_i, res_dbl=res_d, gate_db=np.array([gate]),
+        gate_int=gate_i, gate_dbl=gate_d)
+
+
 if __name__ == "__main__":
     if not refbind.available():
         sys.exit("oracle/_ref/libref.so is missing: run oracle/build_ref.py (needs the reference sources)")
     ldpc()
     ofdm()
+    frames()
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))

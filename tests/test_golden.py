@@ -117,3 +117,57 @@ def test_decode_frame_end_to_end(ofdm, code):
         ok &= r["ok"]
     assert np.array_equal(np.concatenate(info), ofdm["df_info"])
     assert ok == bool(ofdm["df_ok"][0])
+
+
+@pytest.fixture(scope="module")
+def frames():
+    return np.load(os.path.join(G, "frame_small.npz"))
+
+
+FRAME_KEYS = ("detected", "parity_ok", "frame_ok", "frame_id", "bit_errors", "pre_fec_bit_errors", "iterations")
+
+
+def oracle_frame(z, code, f, gate_db=None):
+    """The oracle chain on frame f of the fixture: demod (with SNR / EVM), NMS per
+    block, per-frame aggregation."""
+    phy = _phy(z)
+    start, peak, cfo = z["sync"][f]
+    rx = z["bufs"][f][int(start):int(start) + phy.frame_len]
+    iters, norm = z["bp"]
+    llr, snr, evm = oracle.demod_frame(rx, phy, float(cfo), float(z["noise_var"][0]), stats=True)
+    n = code.n
+    par, its, info = [], [], []
+    for b in range(phy.blocks):
+        r = code.nms(llr[b * n:(b + 1) * n], max_iter=int(iters), alpha=float(norm))
+        par.append(r["ok"])
+        its.append(r["iterations"])
+        info.append(r["hard"][code.info_cols])
+    info = np.concatenate(info)
+    ref_coded = code.encode(z["ref_info"][f].reshape(phy.blocks, code.k)).reshape(-1)
+    gated = gate_db is not None and snr < gate_db
+    out = oracle.frame_finish(phy.blocks, n, code.k, par, its, info, llr[:phy.blocks * n], z["ref_info"][f],
+                              ref_coded, gated=gated)
+    out.update(detected=1, snr_db=snr, evm_rms=evm, info=info)
+    return out
+
+
+@pytest.mark.parametrize("gated", [False, True])
+def test_frame_result_matches_reference(frames, code, gated):
+    """Every FrameDecodeResult field of the reference decode_frame (reference-test
+    mode), with and without the SNR gate: integers and decoded bits exact, SNR and
+    EVM to 1e-9 relative (double sums of the same floats)."""
+    z = frames
+    ri, rd = (z["gate_int"], z["gate_dbl"]) if gated else (z["res_int"], z["res_dbl"])
+    gate = float(z["gate_db"][0]) if gated else None
+    for f in range(len(z["bufs"])):
+        o = oracle_frame(z, code, f, gate)
+        for j, key in enumerate(FRAME_KEYS):
+            assert o[key] == ri[f][j], (f, key)
+        assert o["snr_db"] == pytest.approx(rd[f][2], rel=1e-9, abs=1e-12), f
+        assert o["evm_rms"] == pytest.approx(rd[f][3], rel=1e-9), f
+        if not (gated and o["snr_db"] < gate):
+            assert np.array_equal(o["info"], z["info"][f]), f
+    # both outcomes present: decoded and failed frames, gated and ungated ones
+    assert 0 < ri[:, 2].sum() < len(ri)
+    if gated:
+        assert (ri[:, 1] == 0).sum() > (z["res_int"][:, 1] == 0).sum()
