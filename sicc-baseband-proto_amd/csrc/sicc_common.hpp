@@ -14,8 +14,13 @@
 
 namespace sicc {
 
+// A failed HIP call is reported through the C ABI's status code and message;
+// the runtime's per-thread last error is cleared here so that the failure does
+// not surface again in the caller's own later hipGetLastError checks (torch
+// checks it after every kernel launch).
 [[noreturn]] inline void throw_hip(hipError_t e, const char* what)
 {
+    (void)hipGetLastError();
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 

@@ -165,11 +165,17 @@ def test_ragged_info_width_round_trip():
 
 def test_failed_create_is_clean():
     """A budget the device cannot hold fails server creation with an error status
-    (everything allocated before the failure is released), and the next server works."""
+    (everything allocated before the failure is released), the failure does not
+    stay pending in the HIP runtime (torch's next launch check passes), and the
+    next server works."""
+    import torch
+
     print("[case] server: create with an unallocatable budget", flush=True)
     decs, codes = _members()
     with pytest.raises(sa.SiccError):
         sa.Server(decs[:1], budget_bytes=1 << 52, slots=2)
+    x = torch.full((1024,), 3, dtype=torch.int32, device="cuda:0")
+    assert int(x.sum().item()) == 3072
     jobs = _jobs(codes[:1], 3, 8)
     with sa.Server(decs[:1], budget_bytes=64 << 20, slots=2) as srv:
         for j, (mem, llr) in enumerate(jobs):
