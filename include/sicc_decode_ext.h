@@ -60,6 +60,26 @@ int sicc_decode_device(unsigned long long handle, const float* device_llrs, int 
                        unsigned char* d_bp_success, unsigned char* d_used_osd, int* d_iterations,
                        float* d_distances, void* hip_stream);
 
+/* Allocation-free variant of sicc_decode_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs and semantics; the decode's scratch (BP -> OSD
+ * hand-off and counters) lives in a fixed device array of
+ * SICC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of
+ * buffers the library allocates.  Limits: decoders that run OSD take at most
+ * (SICC_NOALLOC_SCRATCH_BYTES - 512) / (4 n + 4) codewords per call (16 256 for
+ * n = 128; sicc_decode_noalloc_capacity reports it), larger batches fail with
+ * CUDA_OSD_STATUS_INVALID_ARGUMENT; only the short-code flooding kernel runs
+ * here (a layered schedule or a long code fails the same way).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define SICC_NOALLOC_SCRATCH_BYTES (8u << 20)
+int sicc_decode_device_noalloc(unsigned long long handle, const float* device_llrs, int codewords,
+                               unsigned char* d_info_bits, unsigned char* d_code_bits, float* d_app_llrs,
+                               unsigned char* d_bp_success, unsigned char* d_used_osd, int* d_iterations,
+                               float* d_distances, void* hip_stream);
+int sicc_decode_noalloc_capacity(unsigned long long handle, int* max_codewords,
+                                 unsigned long long* scratch_bytes);
+
 /* Kernel timing: with timing on, HIP events recorded on the launch stream
  * bracket every BP and OSD kernel launch of the handle; the getter waits for
  * them and returns summed durations (ms) and launch counts since the last

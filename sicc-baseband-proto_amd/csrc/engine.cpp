@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace sicc {
@@ -279,6 +280,34 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         return;
     }
     if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 128, n <= 256)");
+    sc.ensure(use_osd ? num_cw : 0, code_.n);
+    ScratchView v;
+    v.osd_llr = sc.osd_llr.get();
+    v.osd_index = sc.osd_index.get();
+    v.counters = sc.counters.get();
+    v.cap = sc.cap;
+    decode_short(d_llr, num_cw, bp, osd_order, osd, out, s, v);
+}
+
+void Engine::decode_into(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                         const DecodeOut& out, hipStream_t s, const ScratchView& v)
+{
+    if (num_cw <= 0) return;
+    if (!d_llr) throw std::invalid_argument("null LLR pointer");
+    if (uses_long_kernel(bp))
+        throw std::invalid_argument("the allocation-free decode runs the short-code flooding kernel only");
+    if (!v.counters) throw std::invalid_argument("missing scratch counters");
+    if (osd_order >= 0 && (num_cw > v.cap || !v.osd_llr || !v.osd_index))
+        throw std::invalid_argument("batch of " + std::to_string(num_cw) + " codewords exceeds the " +
+                                    std::to_string(v.cap) + "-codeword OSD scratch");
+    decode_short(d_llr, num_cw, bp, osd_order, osd, out, s, v);
+}
+
+void Engine::decode_short(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                          const DecodeOut& out, hipStream_t s, const ScratchView& v)
+{
+    const bool use_osd = osd_order >= 0;
+    if (use_osd && !osd_ok_) throw std::invalid_argument("OSD is not supported for this code shape (k <= 128, n <= 256)");
     BpIo io;
     io.llr = d_llr;
     io.num_cw = num_cw;
@@ -292,15 +321,14 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
     // counters: [0] OSD hand-off slots, [1] work-stealing cursor; cleared by a
     // kernel (no memset node when this sequence is captured in a graph)
     const bool steal = tune_.steal_chunk > 0;
-    sc.ensure(use_osd ? num_cw : 0, code_.n);
-    if (use_osd || steal) launch_zero_i32(sc.counters.get(), 2, s);
+    if (use_osd || steal) launch_zero_i32(v.counters, 2, s);
     if (use_osd) {
-        io.osd_llr = sc.osd_llr.get();
-        io.osd_index = sc.osd_index.get();
-        io.osd_count = sc.counters.get();
+        io.osd_llr = v.osd_llr;
+        io.osd_index = v.osd_index;
+        io.osd_count = v.counters;
     }
     if (steal) {
-        io.work_ctr = sc.counters.get() + 1;
+        io.work_ctr = v.counters + 1;
         io.chunk = tune_.steal_chunk;
     }
     timed_begin(0, s);
@@ -310,9 +338,9 @@ void Engine::decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_
         OsdParams op = osd;
         op.order = osd_order;
         OsdIo oio;
-        oio.llr = sc.osd_llr.get();
-        oio.index = sc.osd_index.get();
-        oio.count = sc.counters.get();
+        oio.llr = v.osd_llr;
+        oio.index = v.osd_index;
+        oio.count = v.counters;
         oio.code_bits = out.code;
         oio.info_bits = out.info;
         oio.distance = out.distance;

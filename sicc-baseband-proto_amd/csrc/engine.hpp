@@ -44,6 +44,15 @@ struct LongTables {
     BpLongCodeDev dev;
 };
 
+// Scratch of the short-code kernel path held by the caller (no allocation):
+// the BP -> OSD hand-off for up to `cap` codewords and two counters.
+struct ScratchView {
+    float* osd_llr = nullptr;   // [cap][n]
+    int* osd_index = nullptr;   // [cap]
+    int* counters = nullptr;    // [2]
+    int cap = 0;
+};
+
 class Engine {
 public:
     explicit Engine(LdpcCode code);
@@ -59,6 +68,11 @@ public:
     // layered schedules and longer codes run the long-code kernel (no OSD).
     void decode(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
                 const DecodeOut& out, hipStream_t s, DecodeScratch* scratch = nullptr);
+    // decode() on caller-held scratch: allocates nothing.  Only the short-code
+    // (flooding) kernel path; throws std::invalid_argument when `bp` selects the
+    // long-code kernel or OSD needs more than v.cap hand-off slots.
+    void decode_into(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                     const DecodeOut& out, hipStream_t s, const ScratchView& v);
     // OSD on every codeword (no BP).
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 
@@ -108,6 +122,8 @@ private:
 
     void decode_long(const float* d_llr, int num_cw, const BpParams& bp, bool layered, const DecodeOut& out,
                      hipStream_t s, DecodeScratch& sc);
+    void decode_short(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
+                      const DecodeOut& out, hipStream_t s, const ScratchView& v);
     const LongTables& long_tables(bool layered);
     BpLongScratch long_scratch(const LongTables& t, const BpParams& bp, bool layered, DecodeScratch& sc);
 
