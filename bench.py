@@ -191,6 +191,57 @@ def leg_headline(a, sa, np, torch, stream):
     return res
 
 
+def clopper_pearson(k, n, conf=0.95):
+    """Exact binomial confidence interval for k events in n trials."""
+    from scipy.stats import beta
+
+    a = 1.0 - conf
+    lo = 0.0 if k == 0 else float(beta.ppf(a / 2, k, n - k + 1))
+    hi = 1.0 if k == n else float(beta.ppf(1 - a / 2, k + 1, n - k))
+    return lo, hi
+
+
+# --------------------------------------------------------------------- cfg1 --
+def leg_cfg1(a, sa, np, torch, stream, dec):
+    """The plain-decode workload as stated: 10 000 (128,64) codewords at 3 dB,
+    BP (alpha 0.75, 50 iterations) + order-1 OSD, one launch per batch of 256
+    codewords (inputs resident in HBM, launches back to back, one host sync at
+    the end), FER with its Clopper-Pearson 95 % interval; beside it the same
+    frames in one launch."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1010)
+    F, PB = a.cfg1_frames, 256
+    info, llr = gen_llr(dec.generator(), F, 3.0, 0.5, rng)
+    d_llr = torch.from_numpy(llr).to(dev)
+    d_info = torch.empty((F, dec.k), dtype=torch.uint8, device=dev)
+    d_used = torch.empty(F, dtype=torch.uint8, device=dev)
+    sh = stream.cuda_stream
+
+    def per_launch():
+        for s0 in range(0, F, PB):
+            c = min(PB, F - s0)
+            dec.decode_device(d_llr.data_ptr() + s0 * dec.n * 4, c, d_info=d_info.data_ptr() + s0 * dec.k,
+                              d_used=d_used.data_ptr() + s0, stream=sh)
+
+    def one_launch():
+        dec.decode_device(d_llr.data_ptr(), F, d_info=d_info.data_ptr(), d_used=d_used.data_ptr(), stream=sh)
+
+    per_launch()
+    torch.cuda.synchronize()
+    ms = timed(per_launch, 5, stream)
+    got = d_info.cpu().numpy()
+    fe = int((got != info).any(axis=1).sum())
+    one_ms = timed(one_launch, 5, stream)
+    assert np.array_equal(d_info.cpu().numpy(), got)
+    lo, hi = clopper_pearson(fe, F)
+    return {"frames": F, "ebn0_db": 3.0, "launch_codewords": PB, "launches": (F + PB - 1) // PB,
+            "osd_order": 1, "ms": round(ms, 4), "value": round(F * dec.k / (ms * 1e-3) / 1e6, 2),
+            "unit": "info Mbps", "fer": fe / F, "frame_errors": fe, "fer_ci95": [lo, hi],
+            "bit_errors": int((got != info).sum()), "osd_frames": int(d_used.sum().item()),
+            "one_launch_ms": round(one_ms, 4),
+            "one_launch_value": round(F * dec.k / (one_ms * 1e-3) / 1e6, 2)}
+
+
 # -------------------------------------------------------------- calibration --
 def calibrate_mix(a, sa, np, torch):
     """Frame error rate of every mixed-workload code over an Eb/N0 grid (LLRs drawn
@@ -430,8 +481,12 @@ def leg_ofdm(a, sa, np, torch, stream):
     d_llr = torch.empty((F, blocks * n), dtype=torch.float32, device=dev)
     d_info = torch.empty((F * blocks, dec.k), dtype=torch.uint8, device=dev)
     d_ok = torch.empty(F * blocks, dtype=torch.uint8, device=dev)
+    d_ref = torch.from_numpy(info.reshape(F, blocks * dec.k)).to(dev)
+    d_res = torch.empty(F * sa.FRAME_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     sh = stream.cuda_stream
     rxr = sa.OfdmReceiver(phy, n, blocks, max_search=256, noise_var=nvf, threshold=OFDM_SYNC_THRESHOLD)
+    fd = sa.FrameDecoder(rxr, dec)
+    fd.reserve(F)
     torch.cuda.synchronize()
 
     def receive():
@@ -440,28 +495,46 @@ def leg_ofdm(a, sa, np, torch, stream):
     def decode():
         dec.decode_device(d_llr.data_ptr(), F * blocks, d_info=d_info.data_ptr(), d_ok=d_ok.data_ptr(), stream=sh)
 
+    def frame_decode():
+        # sync + demod + BP over every block + per-frame results (FrameDecodeResult) in one call
+        fd.receive_decode_device(d_rx.data_ptr(), F, span, d_info.data_ptr(), d_res.data_ptr(),
+                                 d_ref=d_ref.data_ptr(), d_sync=d_sync.data_ptr(), d_llr=d_llr.data_ptr(),
+                                 stream=sh)
+
     receive()
     torch.cuda.synchronize()
     dec.autotune(d_llr.data_ptr(), F * blocks)
     decode()
+    frame_decode()
     torch.cuda.synchronize()
     rx_ms = timed(receive, a.ofdm_reps, stream)
     dec_ms = timed(decode, a.ofdm_reps, stream)
-    both_ms = timed(lambda: (receive(), decode()), a.ofdm_reps, stream)
+    frame_ms = timed(frame_decode, a.ofdm_reps, stream)
     got = d_info.cpu().numpy()
+    res = d_res.cpu().numpy().view(sa.FRAME_RESULT_DTYPE)
     sync = d_sync.cpu().numpy().view(sa.SYNC_DTYPE)
     fe = int((got != info).reshape(F, -1).any(axis=1).sum())
+    ok = int(res["frame_ok"].sum())
+    info_bits = F * blocks * dec.k
     alg_bytes = F * (8 * span + 4 * blocks * n)
     out = {"frames": F, "codewords": F * blocks, "code": f"QC n={n} k={dec.k}", "modulation": "16-QAM",
            "sync_threshold": OFDM_SYNC_THRESHOLD,
-           "rx_ms": round(rx_ms, 4), "decode_ms": round(dec_ms, 4), "chain_ms": round(both_ms, 4),
-           "frames_per_s": round(F / (both_ms * 1e-3), 1),
-           "value": round(F * blocks * dec.k / (both_ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
-           "frame_error_rate": round(fe / F, 5), "sync_found": int((sync["start"] >= 0).sum()),
+           "rx_ms": round(rx_ms, 4), "decode_ms": round(dec_ms, 4), "frame_ms": round(frame_ms, 4),
+           "frames_per_s": round(F / (frame_ms * 1e-3), 1),
+           "value": round(info_bits / (frame_ms * 1e-3) / 1e6, 2), "unit": "info Mbps processed",
+           "goodput_mbps": round(ok * blocks * dec.k / (frame_ms * 1e-3) / 1e6, 2),
+           "frame_error_rate": round(fe / F, 5), "frames_ok": ok,
+           "frames_detected": int(res["detected"].sum()), "sync_found": int((sync["start"] >= 0).sum()),
+           "mean_snr_db": round(float(res["snr_db"][res["detected"] == 1].mean()), 3),
+           "entry_point": "sicc_frame_receive_decode_device (FrameDecodeResult per frame, reference payload given)",
            "rx_roofline": {"bound": "hbm", "achieved": round(alg_bytes / (rx_ms * 1e-3) / 1e9, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(alg_bytes / (rx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                            "bytes_per_launch": alg_bytes}}
+    if a.cpu_ofdm_frames > 0:
+        out["cpu_baseline"] = cpu_ofdm_chain(a, np, phy, d_rx, span, n, rp, ci, nvf, info.reshape(F, -1),
+                                             res, got.reshape(F, -1))
+    fd.close()
     rxr.close()
     # device-resident test IQ: the batched transmitter (encode + grid + IFFT + compose)
     tphy = wf.OfdmPhy(bits_per_symbol=4, active_sc=512, num_symbols=14, pilot_symbols=[0])
@@ -505,6 +578,7 @@ def leg_osd(a, sa, np, torch, stream):
     gen = dec.generator()
     sh = stream.cuda_stream
     pts = []
+    cpu = None
     for ebn0 in (1.0, 1.5, 2.0, 2.5):
         info, llr = gen_llr(gen, F, ebn0, 0.5, rng)
         d_llr = torch.from_numpy(llr).to(dev)
@@ -523,9 +597,13 @@ def leg_osd(a, sa, np, torch, stream):
         pts.append({"ebn0_db": ebn0, "osd_frames": used, "fer": round(fe / F, 5), "ms": round(ms, 3),
                     "osd_kernel_ms": round(osd_ms, 3), "osd_per_s": round(used / (osd_ms * 1e-3), 1),
                     "value": round(F * 64 / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps"})
+        if ebn0 == 1.5 and a.cpu_osd_frames > 0:
+            cpu = cpu_osd(a, np, gen, llr, got, d_used.cpu().numpy())
+            cpu["ebn0_db"] = ebn0
         del d_llr
     dec.close()
-    return {"frames_per_point": F, "bp_iterations": 20, "osd_order": 2, "teps": 2081, "points": pts}
+    return {"frames_per_point": F, "bp_iterations": 20, "osd_order": 2, "teps": 2081, "points": pts,
+            "cpu_baseline": cpu}
 
 
 # ------------------------------------------------------------------ long ----
@@ -581,6 +659,96 @@ def leg_long(a, sa, np, torch, stream):
 
 
 # ------------------------------------------------------------------- cpu ----
+def host_threads():
+    """Worker threads for the CPU baselines: the host's CPU share for this GPU
+    (OMP_NUM_THREADS where the host sets it, as the GPU hosts do: 16 per GPU),
+    else every core; the core count (nproc) is reported beside it."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_ofdm_chain(a, np, phy, d_rx, span, n, rp, ci, nv, ref, res, got):
+    """The receive workload through the CPU oracle chain (find_frame, demod with
+    SNR / EVM, min-sum per block, per-frame aggregation: LINK:5123-5212,
+    5331-5591) on the first frames of the batch, one thread and the host's CPU
+    share; agreement with the GPU frame results is reported."""
+    import concurrent.futures as cf
+
+    import oracle
+
+    S = min(a.cpu_ofdm_frames, d_rx.shape[0])
+    rx = d_rx[:S].cpu().numpy().view(np.complex64)
+    code = oracle.Code(n, len(rp) - 1, rp, ci, True)
+    blocks = ref.shape[1] // code.k
+
+    def front(f):
+        ok, s, peak, cfo, _ = oracle.find_frame(rx[f], phy.pre_half_len, phy.preamble, min(16, phy.cp),
+                                                OFDM_SYNC_THRESHOLD, phy.fs, max_search=256)
+        if not ok:
+            return None
+        llr, snr, evm = oracle.demod_frame(rx[f][s:s + phy.frame_len], phy, cfo, nv, stats=True)
+        return llr[: blocks * n]
+
+    def chain(threads):
+        t0 = time.perf_counter()
+        if threads == 1:
+            fr = [front(f) for f in range(S)]
+        else:
+            with cf.ThreadPoolExecutor(threads) as ex:
+                fr = list(ex.map(front, range(S)))
+        llr = np.stack([x if x is not None else np.zeros(blocks * n, np.float32) for x in fr])
+        o = code.bposd(llr.reshape(-1, n), max_iter=30, alpha=0.75, order=-1, threads=threads)
+        fin = []
+        for f in range(S):
+            par = o["bp_ok"].reshape(S, blocks)[f]
+            it = o["iterations"].reshape(S, blocks)[f]
+            inf = o["info"].reshape(S, -1)[f]
+            rc = code.encode(ref[f].reshape(blocks, code.k)).reshape(-1)
+            fin.append(oracle.frame_finish(blocks, n, code.k, par, it, inf, llr[f], ref[f], rc)
+                       if fr[f] is not None else None)
+        return time.perf_counter() - t0, o["info"].reshape(S, -1), fin
+
+    t1, info1, fin = chain(1)
+    T = host_threads()
+    tm, _, _ = chain(T)
+    det = [x is not None for x in fin]
+    agree = all(np.array_equal(info1[f], got[f]) and fin[f]["frame_ok"] == res["frame_ok"][f]
+                for f in range(S) if det[f])
+    bits = S * ref.shape[1]
+    return {"value": round(S / t1, 2), "unit": "frames/s", "info_mbps": round(bits / t1 / 1e6, 4), "cores": 1,
+            "kind": "port", "sample": f"first {S} frames of the receive batch through the oracle chain "
+                                      "(find_frame, demod, min-sum 30 it per block, aggregation)",
+            "all_cores": {"value": round(S / tm, 2), "unit": "frames/s", "info_mbps": round(bits / tm / 1e6, 4),
+                          "cores": T, "nproc": os.cpu_count()},
+            "agrees_with_gpu": bool(agree)}
+
+
+def cpu_osd(a, np, gen, llr, got, used):
+    """Order-2 OSD workload (20 BP iterations, OSD on every BP failure) through
+    the CPU oracle on a sample of one Eb/N0 point, one thread and the host's CPU
+    share."""
+    import oracle
+
+    sa_ = sys.modules["sicc_amd"]
+    n_, m_, rp, ci = oracle.alist_parse(sa_.CCSDS_128_64)
+    code = oracle.Code(n_, m_, rp, ci, True)
+    S = min(a.cpu_osd_frames, llr.shape[0])
+    t0 = time.perf_counter()
+    r1 = code.bposd(llr[:S], max_iter=20, alpha=0.75, order=2, max_flip=64, max_tests=4096, threads=1)
+    t1 = time.perf_counter() - t0
+    T = host_threads()
+    t0 = time.perf_counter()
+    code.bposd(llr[:S], max_iter=20, alpha=0.75, order=2, max_flip=64, max_tests=4096, threads=T)
+    tm = time.perf_counter() - t0
+    osd = int(r1["used_osd"].sum())
+    return {"value": round(osd / t1, 1), "unit": "OSD frames/s", "info_mbps": round(S * 64 / t1 / 1e6, 4),
+            "cores": 1, "kind": "port",
+            "sample": f"first {S} frames of the point, {osd} to OSD; oracle BP (20 it) + order-2 OSD in C",
+            "all_cores": {"value": round(osd / tm, 1), "unit": "OSD frames/s",
+                          "info_mbps": round(S * 64 / tm / 1e6, 4), "cores": T, "nproc": os.cpu_count()},
+            "agrees_with_gpu": bool(np.array_equal(r1["info"], got[:S]) and np.array_equal(r1["used_osd"], used[:S]))}
+
+
 def cpu_baselines(a, sa, np, llr, got):
     import oracle
     import refbind
@@ -591,14 +759,14 @@ def cpu_baselines(a, sa, np, llr, got):
     tc = time.perf_counter()
     res = ocode.bposd(llr[:S], max_iter=50, alpha=0.75, order=a.osd_order, max_flip=64, threads=1)
     cpu1_s = time.perf_counter() - tc
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     SM = min(llr.shape[0], 8 * S)
     tc = time.perf_counter()
     resm = ocode.bposd(llr[:SM], max_iter=50, alpha=0.75, order=a.osd_order, max_flip=64, threads=threads)
     cpum_s = time.perf_counter() - tc
     cpu = {"value": round(S * 64 / cpu1_s / 1e6, 3), "unit": "info Mbps", "cores": 1, "kind": "port",
            "sample": f"first {S} codewords of the headline batch; oracle BP+OSD in C, one thread",
-           "all_cores": {"value": round(SM * 64 / cpum_s / 1e6, 3), "cores": threads,
+           "all_cores": {"value": round(SM * 64 / cpum_s / 1e6, 3), "cores": threads, "nproc": os.cpu_count(),
                          "sample": f"first {SM} codewords, OpenMP over codewords"},
            "agrees_with_gpu": bool(np.array_equal(res["info"], got[:S]) and np.array_equal(resm["info"], got[:SM]))}
     ref = None
@@ -622,6 +790,7 @@ def main() -> int:
     ap.add_argument("--ebn0", type=float, default=4.0)
     ap.add_argument("--osd-order", type=int, default=1)
     ap.add_argument("--plain-batches", type=int, default=400)
+    ap.add_argument("--cfg1-frames", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--served-jobs", type=int, default=20000)
     ap.add_argument("--served-mix-jobs", type=int, default=5000)
@@ -633,11 +802,15 @@ def main() -> int:
     ap.add_argument("--mixed-reps", type=int, default=5)
     ap.add_argument("--ofdm-frames", type=int, default=4096)
     ap.add_argument("--ofdm-reps", type=int, default=5)
+    ap.add_argument("--cpu-ofdm-frames", type=int, default=24,
+                    help="frames of the receive workload through the CPU oracle chain (0: skip)")
+    ap.add_argument("--cpu-osd-frames", type=int, default=6000,
+                    help="frames of the order-2 OSD workload through the CPU oracle (0: skip)")
     ap.add_argument("--osd-frames", type=int, default=262144)
     ap.add_argument("--long-frames", type=int, default=4096)
     ap.add_argument("--long-reps", type=int, default=3)
-    ap.add_argument("--legs", default="served,mixed,ofdm,osd,long,cpu",
-                    help="comma list of extra legs: served, mixed, ofdm, osd, long, cpu ('' = headline only)")
+    ap.add_argument("--legs", default="cfg1,served,mixed,ofdm,osd,long,cpu",
+                    help="comma list of extra legs: cfg1, served, mixed, ofdm, osd, long, cpu ('' = headline only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-mix", type=int, default=0,
                     help="frames per point: print the FER curves of the mixed-workload codes and exit")
@@ -670,6 +843,9 @@ def main() -> int:
     h = leg_headline(a, sa, np, torch, stream)
     dec128, llr128, got128 = h.pop("_keep")
     extra = {}
+    if "cfg1" in legs:
+        log("cfg1")
+        extra["cfg1"] = leg_cfg1(a, sa, np, torch, stream, dec128)
     if "served" in legs:
         log("served")
         extra["served"] = leg_served(a, sa, np, torch, dec128)

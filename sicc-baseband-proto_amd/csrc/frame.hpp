@@ -27,7 +27,8 @@ struct FrameFinishArgs {
     const uint8_t* ref_info = nullptr;    // [frames][blocks * k] reference payload
     const float* llr = nullptr;           // [frames][blocks * n]
     // generator columns c0 .. n - 1 (c0 = k for a systematic-front code, whose
-    // first k columns are the identity): bit r % 64 of gen_cols[(c - c0) * kw + r / 64] is G[r][c]
+    // first k columns are the identity), word-major: bit r % 64 of
+    // gen_cols[(r / 64) * (n - c0) + (c - c0)] is G[r][c]
     const unsigned long long* gen_cols = nullptr;
     int kw = 0;
     int c0 = 0;

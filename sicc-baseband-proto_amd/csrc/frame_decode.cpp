@@ -36,7 +36,7 @@ struct FrameDec {
     int k = 0;
     double gate = -1e300;
     hipStream_t stream = nullptr;
-    // generator columns for the pre-FEC count (built on first use)
+    // generator columns for the pre-FEC count, word-major (built on first use)
     DevBuf<unsigned long long> gen;
     int kw = 0;
     int c0 = 0;
@@ -108,7 +108,7 @@ struct FrameDec {
         for (int r = 0; r < c.k; ++r)
             for (int col = c0; col < c.n; ++col)
                 if (g[static_cast<size_t>(r) * c.n + col])
-                    cols[static_cast<size_t>(col - c0) * kw + r / 64] |= 1ull << (r % 64);
+                    cols[static_cast<size_t>(r / 64) * (c.n - c0) + (col - c0)] |= 1ull << (r % 64);
         gen.alloc(cols.size());
         SICC_HIP(hipMemcpy(gen.get(), cols.data(), cols.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
         gen_lds = frame_gen_fits_lds(c.n - c0, kw);
