@@ -275,7 +275,12 @@ struct MultiDec {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<sicc_group_t> key;
-    std::vector<const void*> key_tunes;
+    std::vector<unsigned char> key_params;  // every group's member parameters the graph bakes in
+    // Per-group scratch (BP -> OSD hand-off, counters, long-code messages): groups
+    // run concurrently, also several groups of one member, so none shares a
+    // member's own scratch.  The graph's set is sized before capture and only
+    // replaced together with the graph; eager calls use their own set.
+    std::vector<std::unique_ptr<DecodeScratch>> graph_scratch, eager_scratch;
 
     ~MultiDec()
     {
@@ -305,9 +310,33 @@ bool same_groups(const std::vector<sicc_group_t>& a, const sicc_group_t* b, int 
     return true;
 }
 
-// Enqueue every group on its own lane stream, forked from and joined back to `s`.
-void enqueue_groups(MultiDec& md, const sicc_group_t* groups, int n, hipStream_t s)
+// Bytes of everything a captured decode of these groups depends on besides the
+// group descriptors: each member's BP parameters, OSD order and parameters and
+// BP launch shape.
+std::vector<unsigned char> group_params(const MultiDec& md, const sicc_group_t* groups, int n)
 {
+    std::vector<unsigned char> k;
+    auto put = [&k](const void* p, size_t b) {
+        const auto* c = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), c, c + b);
+    };
+    for (int i = 0; i < n; ++i) {
+        const Decoder& d = *md.members[groups[i].member];
+        const BpTune& t = d.eng->tune();
+        put(&d.bp, sizeof(d.bp));
+        put(&d.osd_order, sizeof(d.osd_order));
+        put(&d.osd, sizeof(d.osd));
+        put(&t, sizeof(t));
+    }
+    return k;
+}
+
+// Enqueue every group on its own lane stream, forked from and joined back to
+// `s`; group i decodes with scratch[i].
+void enqueue_groups(MultiDec& md, const sicc_group_t* groups, int n, hipStream_t s,
+                    std::vector<std::unique_ptr<DecodeScratch>>& scratch)
+{
+    while (static_cast<int>(scratch.size()) < n) scratch.push_back(std::make_unique<DecodeScratch>());
     SICC_HIP(hipEventRecord(md.fork, s));
     for (int i = 0; i < n; ++i) {
         const sicc_group_t& g = groups[i];
@@ -319,7 +348,8 @@ void enqueue_groups(MultiDec& md, const sicc_group_t* groups, int n, hipStream_t
         out.bp_ok = g.d_bp_ok;
         out.iterations = g.d_iterations;
         out.used_osd = g.d_used_osd;
-        d.eng->decode(g.d_llr, g.count, d.bp, d.kind == Kind::Stream ? d.osd_order : -1, d.osd, out, ls);
+        d.eng->decode(g.d_llr, g.count, d.bp, d.kind == Kind::Stream ? d.osd_order : -1, d.osd, out, ls,
+                      scratch[i].get());
         SICC_HIP(hipEventRecord(md.joins[i % md.joins.size()], ls));
     }
     const int used = std::min<int>(n, static_cast<int>(md.lanes.size()));
@@ -869,30 +899,29 @@ int sicc_multi_decode_device(unsigned long long handle, const sicc_group_t* grou
         }
         hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : md->main;
         if (!use_graph) {
-            enqueue_groups(*md, groups, n_groups, s);
+            enqueue_groups(*md, groups, n_groups, s, md->eager_scratch);
             return CUDA_OSD_STATUS_OK;
         }
-        std::vector<const void*> tunes;
-        for (int i = 0; i < n_groups; ++i) {
-            const BpTune& t = md->members[groups[i].member]->eng->tune();
-            tunes.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(
-                (t.team << 24) ^ (t.teams_per_block << 20) ^ (t.max_waves << 4) ^ t.steal_chunk)));
-        }
-        if (!md->exec || !same_groups(md->key, groups, n_groups) || tunes != md->key_tunes) {
+        const std::vector<unsigned char> params = group_params(*md, groups, n_groups);
+        if (!md->exec || !same_groups(md->key, groups, n_groups) || params != md->key_params) {
             if (md->exec) SICC_HIP(hipGraphExecDestroy(md->exec));
             if (md->graph) SICC_HIP(hipGraphDestroy(md->graph));
             md->exec = nullptr;
             md->graph = nullptr;
+            // an earlier replay may still read the old graph's scratch
+            SICC_HIP(hipStreamSynchronize(s));
+            SICC_HIP(hipStreamSynchronize(md->main));
+            md->graph_scratch.clear();
             for (int i = 0; i < n_groups; ++i) {
                 Decoder& d = *md->members[groups[i].member];
                 d.eng->set_timing(false);
-                if ((d.kind == Kind::Stream && d.osd_order >= 0) || d.eng->uses_long_kernel(d.bp))
-                    d.eng->reserve(groups[i].count, &d.bp);
+                md->graph_scratch.push_back(std::make_unique<DecodeScratch>());
+                const bool osd = d.kind == Kind::Stream && d.osd_order >= 0;
+                d.eng->reserve(osd ? groups[i].count : 0, &d.bp, md->graph_scratch.back().get());
             }
-            SICC_HIP(hipStreamSynchronize(md->main));
             SICC_HIP(hipStreamBeginCapture(md->main, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_groups(*md, groups, n_groups, md->main);
+                enqueue_groups(*md, groups, n_groups, md->main, md->graph_scratch);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(md->main, &g);
@@ -902,7 +931,7 @@ int sicc_multi_decode_device(unsigned long long handle, const sicc_group_t* grou
             SICC_HIP(hipStreamEndCapture(md->main, &md->graph));
             SICC_HIP(hipGraphInstantiate(&md->exec, md->graph, nullptr, nullptr, 0));
             md->key.assign(groups, groups + n_groups);
-            md->key_tunes = tunes;
+            md->key_params = params;
         }
         SICC_HIP(hipGraphLaunch(md->exec, s));
         return CUDA_OSD_STATUS_OK;

@@ -137,3 +137,59 @@ def test_full_mixed_batch():
             assert np.array_equal(info[i], o["hard"][code.info_cols]), (code.n, i)
     for dd in decs:
         dd.close()
+
+
+def test_two_groups_of_one_member_and_parameter_changes():
+    """Several groups of the same BP-OSD member in one call (eager and graph)
+    decode concurrently with separate scratch (OSD hand-off, counters, stealing
+    cursor): each equals the oracle.  A member's OSD order changed between
+    replays is picked up by the next replay (the graph is recaptured), which
+    then equals the eager decode with the new order."""
+    import torch
+
+    print("[case] mixed: two groups of one member, OSD order change between replays", flush=True)
+    n, m, rp, ci = oracle.alist_parse(sa.CCSDS_128_64)
+    code = oracle.Code(n, m, rp, ci, True)
+    rng = np.random.default_rng(13)
+    counts = [3000, 1700, 2500]
+    llrs = [oracle.awgn_llr(code.encode(rng.integers(0, 2, (c, code.k), dtype=np.uint8)), 2.0, 0.5, rng)
+            for c in counts]
+    data = [{"llr": l, "d_llr": torch.from_numpy(l).cuda(),
+             "d_info": torch.zeros((l.shape[0], code.k), dtype=torch.uint8, device="cuda"),
+             "d_ok": torch.zeros(l.shape[0], dtype=torch.uint8, device="cuda"),
+             "d_it": torch.zeros(l.shape[0], dtype=torch.int32, device="cuda"),
+             "d_used": torch.zeros(l.shape[0], dtype=torch.uint8, device="cuda")} for l in llrs]
+
+    def groups():
+        return [sa.Group(0, d["llr"].shape[0], d["d_llr"].data_ptr(), d["d_info"].data_ptr(), d["d_ok"].data_ptr(),
+                         d["d_it"].data_ptr(), d["d_used"].data_ptr()) for d in data]
+
+    def check(order):
+        for d in data:
+            o = code.bposd(d["llr"], max_iter=50, alpha=0.75, order=order, max_flip=64)
+            assert np.array_equal(d["d_info"].cpu().numpy(), o["info"]), order
+            assert np.array_equal(d["d_used"].cpu().numpy(), o["used_osd"]), order
+            assert np.array_equal(d["d_it"].cpu().numpy(), o["iterations"]), order
+
+    def clear():
+        for d in data:
+            d["d_info"].fill_(9)
+            d["d_used"].fill_(9)
+            d["d_it"].fill_(-1)
+        torch.cuda.synchronize()
+
+    with sa.Decoder.stream(sa.stream_config(max_iter=50, alpha=0.75)) as dec:
+        dec.set_osd(1, 64, 4096)
+        dec.set_tune(1, 4, 0, 16)  # work stealing: the cursor is per-group scratch too
+        with sa.MultiDecoder([dec]) as md:
+            for use_graph in (False, True, True):
+                clear()
+                md.decode(groups(), use_graph=use_graph)
+                torch.cuda.synchronize()
+                check(1)
+            dec.set_osd(2, 64, 4096)
+            for use_graph in (True, False):
+                clear()
+                md.decode(groups(), use_graph=use_graph)
+                torch.cuda.synchronize()
+                check(2)
