@@ -26,6 +26,7 @@ struct OfdmSyncArgs {
     double threshold = 0.45;
     double fs = 15.36e6;
     const float2* preamble = nullptr;  // [2 * half_len]
+    const double2* preamble_d = nullptr;  // the same samples in double
     double pre_energy = 0.0;           // sum of |preamble|^2 (float norms, double sum)
     OfdmSyncResult* out = nullptr;     // [frames]
     int window_in_lds = 1;             // set by the launcher
@@ -56,6 +57,7 @@ struct OfdmDemodArgs {
     const double2* twiddle = nullptr;       // [nfft] exp(-2 pi i m / nfft)
     float noise_var = 1e-3f;
     double noise_var_d = 1e-3;
+    double inv_noise_var_d = 1e3;  // 1 / noise_var_d
     double fs = 15.36e6;
     int use_pilots = 1;
     float* llr = nullptr;  // [frames][llr_per_frame]

@@ -165,6 +165,11 @@ void OfdmRx::init(const float2* preamble, const std::vector<int>& nat, const std
         if (count) SICC_HIP(hipMemcpy(buf.get(), src, count * sizeof(*src), hipMemcpyHostToDevice));
     };
     up(preamble_, preamble, cfg.preamble_len);
+    {
+        std::vector<double2> pd(cfg.preamble_len);
+        for (int i = 0; i < cfg.preamble_len; ++i) pd[i] = make_double2(preamble[i].x, preamble[i].y);
+        up(preamble_d_, pd.data(), pd.size());
+    }
     up(pilot_grid_, pilot_grid, static_cast<size_t>(n_pilot_) * cfg.active_sc);
     up(constellation_, con.data(), con.size());
     up(pilot_symbols_, ps.data(), ps.size());
@@ -193,6 +198,7 @@ void OfdmRx::sync(const float2* d_rx, int frames, int span, OfdmSyncResult* d_ou
     a.threshold = cfg_.sync_threshold;
     a.fs = cfg_.sample_rate;
     a.preamble = preamble_.get();
+    a.preamble_d = preamble_d_.get();
     a.pre_energy = pre_energy_;
     a.out = d_out;
     launch_ofdm_sync(a, frames, s);
@@ -228,6 +234,7 @@ void OfdmRx::demod(const float2* d_rx, int frames, int span, const OfdmSyncResul
     const double nv = std::max(static_cast<double>(cfg_.demod_noise_var), 1e-8);
     a.noise_var = static_cast<float>(nv);
     a.noise_var_d = nv;
+    a.inv_noise_var_d = 1.0 / nv;
     a.fs = cfg_.sample_rate;
     a.use_pilots = cfg_.use_pilot_channel_est;
     a.llr = d_llr;
@@ -421,10 +428,14 @@ int cuda_osd_find_frame(const double* signal_real, const double* signal_imag, in
         }
         hipStream_t s = nullptr;
         SICC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        std::vector<double2> pred(preamble_length);
+        for (int i = 0; i < preamble_length; ++i) pred[i] = make_double2(pre[i].x, pre[i].y);
         DevBuf<float2> d_sig(signal_length), d_pre(preamble_length);
+        DevBuf<double2> d_pred(preamble_length);
         DevBuf<OfdmSyncResult> d_out(1);
         SICC_HIP(hipMemcpyAsync(d_sig.get(), sig.data(), sig.size() * sizeof(float2), hipMemcpyHostToDevice, s));
         SICC_HIP(hipMemcpyAsync(d_pre.get(), pre.data(), pre.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+        SICC_HIP(hipMemcpyAsync(d_pred.get(), pred.data(), pred.size() * sizeof(double2), hipMemcpyHostToDevice, s));
         OfdmSyncArgs a;
         a.rx = d_sig.get();
         a.span = signal_length;
@@ -434,6 +445,7 @@ int cuda_osd_find_frame(const double* signal_real, const double* signal_imag, in
         a.threshold = sync_peak_thresh;
         a.fs = fs;
         a.preamble = d_pre.get();
+        a.preamble_d = d_pred.get();
         a.pre_energy = pe;
         a.out = d_out.get();
         launch_ofdm_sync(a, 1, s);

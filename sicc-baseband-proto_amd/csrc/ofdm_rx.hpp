@@ -53,6 +53,7 @@ private:
     int n_data_ = 0;
     double pre_energy_ = 0.0;
     DevBuf<float2> preamble_, pilot_grid_, constellation_;
+    DevBuf<double2> preamble_d_;  // the preamble's float samples widened to double (sync refinement)
     DevBuf<int> pilot_symbols_, data_symbols_, used_natural_, p0_, p1_;
     DevBuf<float> w0_, w1_;
     DevBuf<double2> twiddle_;

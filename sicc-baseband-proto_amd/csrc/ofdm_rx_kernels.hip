@@ -18,8 +18,9 @@
 // elements past blocks * code_n carry pad bits and produce no LLR.
 //
 // Mapping.  Sync: one 256-thread workgroup per frame; the search window sits
-// in LDS, each thread sums whole candidates in FP64 (direct sums, no sliding
-// recurrence), then block argmax / refine / CFO.  Demod: one 256-thread
+// in LDS and the CPU search's sliding recurrence runs as a block scan of
+// per-candidate increments (FP64), then block argmax / refine (one wavefront
+// per candidate, window energies from the scan) / CFO.  Demod: one 256-thread
 // workgroup per (frame, pilot symbol) writes LS estimates (A floats2 per pilot,
 // L2-resident), then one per (frame, data symbol) derotates + strips the CP
 // while loading the FFT input, runs the radix-4 Stockham FFT in FP64 through
@@ -83,73 +84,121 @@ __global__ __launch_bounds__(kThreads) void ofdm_sync_kernel(OfdmSyncArgs a)
     double best_m = 0.0;
     int best_d = 0x7fffffff;
     if (a.prefix) {
-        // Windows staged in LDS: prefix sums S[j] = sum_{i<j} conj(x[i]) x[i+L] and
-        // Q[j] = sum_{i<j} |x[i]|^2 (float norms, double sums), then
-        // P(d) = S[d+L] - S[d], E1 = Q[d+L] - Q[d], E2 = Q[d+2L] - Q[d+L]: O(window)
-        // work per frame instead of O(candidates * L).
-        const int nc = wlen - L;
-        double* Sr = reinterpret_cast<double*>(win + wlen);
-        double* Si = Sr + nc + 1;
-        double* Q = Si + nc + 1;
+        // Windows staged in LDS: the sliding recurrence of the CPU search
+        // (P(d+1) = P(d) + conj(x[d+L]) x[d+2L] - conj(x[d]) x[d+L], E1, E2 by
+        // float differences of float norms, find_frame LINK:5123-5212), evaluated
+        // in parallel: the sums at d = 0 by a block reduction, the per-candidate
+        // increments by a block scan.  The window energies E1 + E2 of every
+        // candidate up to the refinement range stay in LDS for the refinement.
+        const int nd = hi_all + 1;  // candidates d = 0 .. hi_all (the coarse search uses d <= limit)
+        double* Ed = reinterpret_cast<double*>(win + wlen);  // [nd]
         const int t = threadIdx.x;
-        const int per_c = (nc + kThreads - 1) / kThreads;
-        const int per_q = (wlen + kThreads - 1) / kThreads;
-        const int lc = min(nc, t * per_c), hc = min(nc, lc + per_c);
-        const int lq = min(wlen, t * per_q), hq = min(wlen, lq + per_q);
-        double sr = 0.0, si = 0.0, sq = 0.0;
-        for (int i = lc; i < hc; ++i) {
-            const float2 u = w[i], v = w[i + L];
-            sr += static_cast<double>(u.x) * v.x + static_cast<double>(u.y) * v.y;
-            si += static_cast<double>(u.x) * v.y - static_cast<double>(u.y) * v.x;
-        }
-        for (int i = lq; i < hq; ++i) sq += static_cast<double>(w[i].x * w[i].x + w[i].y * w[i].y);
-        bm[t] = sr;
-        __shared__ double s_i[kThreads], s_q[kThreads];
-        s_i[t] = si;
-        s_q[t] = sq;
-        __syncthreads();
-        if (t == 0) {  // exclusive scan of the per-thread totals
-            double ar = 0.0, ai = 0.0, aq = 0.0;
-            for (int u = 0; u < kThreads; ++u) {
-                const double xr = bm[u], xi = s_i[u], xq = s_q[u];
-                bm[u] = ar;
-                s_i[u] = ai;
-                s_q[u] = aq;
-                ar += xr;
-                ai += xi;
-                aq += xq;
+        const int ln = t & 63, wv = t >> 6;
+        __shared__ double base[4];
+        __shared__ double wsum[4 * (kThreads / 64)];
+        {
+            double p0r = 0.0, p0i = 0.0, e10 = 0.0, e20 = 0.0;
+            for (int i = t; i < L; i += kThreads) {
+                const float2 u = w[i], v = w[i + L];
+                const double ur = u.x, ui = u.y, vr = v.x, vi = v.y;
+                p0r += ur * vr + ui * vi;
+                p0i += ur * vi - ui * vr;
+                e10 += static_cast<double>(u.x * u.x + u.y * u.y);
+                e20 += static_cast<double>(v.x * v.x + v.y * v.y);
             }
-            Sr[0] = 0.0;
-            Si[0] = 0.0;
-            Q[0] = 0.0;
+            for (int o = 32; o > 0; o >>= 1) {
+                p0r += __shfl_xor(p0r, o);
+                p0i += __shfl_xor(p0i, o);
+                e10 += __shfl_xor(e10, o);
+                e20 += __shfl_xor(e20, o);
+            }
+            if (ln == 0) {
+                wsum[4 * wv] = p0r;
+                wsum[4 * wv + 1] = p0i;
+                wsum[4 * wv + 2] = e10;
+                wsum[4 * wv + 3] = e20;
+            }
+            __syncthreads();
+            if (t < 4) {
+                double b = 0.0;
+                for (int u = 0; u < kThreads / 64; ++u) b += wsum[4 * u + t];
+                base[t] = b;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        sr = bm[t];
-        si = s_i[t];
-        sq = s_q[t];
-        for (int i = lc; i < hc; ++i) {
-            const float2 u = w[i], v = w[i + L];
-            sr += static_cast<double>(u.x) * v.x + static_cast<double>(u.y) * v.y;
-            si += static_cast<double>(u.x) * v.y - static_cast<double>(u.y) * v.x;
-            Sr[i + 1] = sr;
-            Si[i + 1] = si;
+        // thread t owns candidates [lc, hc); the increment of d moves d -> d + 1
+        const int per_c = (nd + kThreads - 1) / kThreads;
+        const int lc = min(nd, t * per_c), hc = min(nd, lc + per_c);
+        auto incr = [&](int d, double& dr, double& di, double& d1, double& d2) {
+            const float2 x0 = w[d], x1 = w[d + L], x2 = w[d + pre];
+            const double x0r = x0.x, x0i = x0.y, x1r = x1.x, x1i = x1.y, x2r = x2.x, x2i = x2.y;
+            dr = (x1r * x2r + x1i * x2i) - (x0r * x1r + x0i * x1i);
+            di = (x1r * x2i - x1i * x2r) - (x0r * x1i - x0i * x1r);
+            const float n0 = x0.x * x0.x + x0.y * x0.y, n1 = x1.x * x1.x + x1.y * x1.y;
+            const float n2 = x2.x * x2.x + x2.y * x2.y;
+            d1 = static_cast<double>(n1 - n0);
+            d2 = static_cast<double>(n2 - n1);
+        };
+        double tr = 0.0, ti = 0.0, t1 = 0.0, t2 = 0.0;  // this thread's increments (all but the last candidate's)
+        for (int d = lc; d < hc && d < nd - 1; ++d) {
+            double dr, di, d1, d2;
+            incr(d, dr, di, d1, d2);
+            tr += dr;
+            ti += di;
+            t1 += d1;
+            t2 += d2;
         }
-        for (int i = lq; i < hq; ++i) {
-            sq += static_cast<double>(w[i].x * w[i].x + w[i].y * w[i].y);
-            Q[i + 1] = sq;
-        }
-        __syncthreads();
-        for (int d = t; d <= limit; d += blockDim.x) {
-            const double pr = Sr[d + L] - Sr[d], pi = Si[d + L] - Si[d];
-            const double e1 = Q[d + L] - Q[d], e2 = Q[d + 2 * L] - Q[d + L];
-            const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
-            const double m = (pr * pr + pi * pi) / den;
-            if (m > best_m) {
-                best_m = m;
-                best_d = d;
+        // exclusive scan of the thread totals: shuffles within each wavefront, then the
+        // totals of the earlier wavefronts
+        double ir = tr, ii = ti, i1 = t1, i2 = t2;
+        for (int o = 1; o < 64; o <<= 1) {
+            const double ur = __shfl_up(ir, o), ui = __shfl_up(ii, o), u1 = __shfl_up(i1, o), u2 = __shfl_up(i2, o);
+            if (ln >= o) {
+                ir += ur;
+                ii += ui;
+                i1 += u1;
+                i2 += u2;
             }
         }
-        __syncthreads();  // bm is reused by the argmax below
+        __syncthreads();  // wsum is reused
+        if (ln == 63) {
+            wsum[4 * wv] = ir;
+            wsum[4 * wv + 1] = ii;
+            wsum[4 * wv + 2] = i1;
+            wsum[4 * wv + 3] = i2;
+        }
+        __syncthreads();
+        double pr = base[0], pi = base[1], e1 = base[2], e2 = base[3];
+        for (int u = 0; u < wv; ++u) {
+            pr += wsum[4 * u];
+            pi += wsum[4 * u + 1];
+            e1 += wsum[4 * u + 2];
+            e2 += wsum[4 * u + 3];
+        }
+        pr += ir - tr;
+        pi += ii - ti;
+        e1 += i1 - t1;
+        e2 += i2 - t2;
+        for (int d = lc; d < hc; ++d) {
+            if (d <= limit) {
+                const double den = e1 * e2 > 1e-18 ? e1 * e2 : 1e-18;
+                const double m = (pr * pr + pi * pi) / den;
+                if (m > best_m) {
+                    best_m = m;
+                    best_d = d;
+                }
+            }
+            Ed[d] = e1 + e2;
+            if (d < nd - 1) {
+                double dr, di, d1, d2;
+                incr(d, dr, di, d1, d2);
+                pr += dr;
+                pi += di;
+                e1 += d1;
+                e2 += d2;
+            }
+        }
+        __syncthreads();  // bm is reused by the argmax below; Ed complete
     } else {
         for (int d = threadIdx.x; d <= limit; d += blockDim.x) {
             double pr = 0.0, pi = 0.0, e1 = 0.0, e2 = 0.0;
@@ -206,21 +255,59 @@ __global__ __launch_bounds__(kThreads) void ofdm_sync_kernel(OfdmSyncArgs a)
     const int nwv = blockDim.x >> 6;
     double wbest = 0.0;
     int wbest_d = 0x7fffffff;
+    // window energies of the candidates when the coarse search left them in LDS
+    const double* Ep = a.prefix ? reinterpret_cast<const double*>(win + wlen) : nullptr;
+    // Preambles of up to 64 * kRefineRegs samples: each lane keeps its preamble
+    // samples in registers for all candidates and issues a candidate's window
+    // reads together; longer ones loop through L2.
+    constexpr int kRefineRegs = 16;
+    const bool regs = pre <= 64 * kRefineRegs;
+    double2 qv[kRefineRegs];
+#pragma unroll
+    for (int j = 0; j < kRefineRegs; ++j) {
+        const int i = lane + 64 * j;
+        qv[j] = (regs && i < pre) ? a.preamble_d[i] : make_double2(0.0, 0.0);
+    }
     for (int d = lo + wv; d <= hi; d += nwv) {
         double cr = 0.0, ci = 0.0, eng = 0.0;
-        for (int i = lane; i < pre; i += 64) {
-            const float2 y = w[d + i];
-            const float2 q = a.preamble[i];
-            const double yr = y.x, yi = y.y, qr = q.x, qi = q.y;
-            cr += yr * qr + yi * qi;
-            ci += yi * qr - yr * qi;
-            eng += static_cast<double>(y.x * y.x + y.y * y.y);
+        // y conj(q) accumulated with fused multiply-adds (the refined metric needs
+        // a few parts in 1e16, not the CPU's separately rounded products)
+        if (regs) {
+            double cr1 = 0.0, ci1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < kRefineRegs; ++j) {
+                const int i = lane + 64 * j;
+                if (i < pre) {
+                    const float2 y = w[d + i];
+                    const double yr = y.x, yi = y.y, qr = qv[j].x, qi = qv[j].y;
+                    if (j & 1) {
+                        cr1 = fma(yi, qi, fma(yr, qr, cr1));
+                        ci1 = fma(-yr, qi, fma(yi, qr, ci1));
+                    } else {
+                        cr = fma(yi, qi, fma(yr, qr, cr));
+                        ci = fma(-yr, qi, fma(yi, qr, ci));
+                    }
+                    if (!Ep) eng += static_cast<double>(y.x * y.x + y.y * y.y);
+                }
+            }
+            cr += cr1;
+            ci += ci1;
+        } else {
+            for (int i = lane; i < pre; i += 64) {
+                const float2 y = w[d + i];
+                const double2 q = a.preamble_d[i];
+                const double yr = y.x, yi = y.y;
+                cr = fma(yi, q.y, fma(yr, q.x, cr));
+                ci = fma(-yr, q.y, fma(yi, q.x, ci));
+                if (!Ep) eng += static_cast<double>(y.x * y.x + y.y * y.y);
+            }
         }
         for (int o = 32; o > 0; o >>= 1) {
             cr += __shfl_xor(cr, o);
             ci += __shfl_xor(ci, o);
-            eng += __shfl_xor(eng, o);
+            if (!Ep) eng += __shfl_xor(eng, o);
         }
+        if (Ep) eng = Ep[d];
         const double den = eng * a.pre_energy > 1e-18 ? eng * a.pre_energy : 1e-18;
         const double m = (cr * cr + ci * ci) / den;
         if (m > wbest) {
@@ -423,14 +510,15 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
             acc[3] += static_cast<double>(cr * cr + ci * ci);
             if (obit >= a.llr_per_frame) continue;  // pad resource elements carry no LLR
         }
-        const float den = h2 + nv;
-        const float er = tr / den, ei = ti / den;
-        double s2 = static_cast<double>(a.noise_var_d) / static_cast<double>(h2);
-        if (s2 < 1e-8) s2 = 1e-8;
-        // one division per resource element: x * (1 / s2) is within 2 double ulps
-        // of x / s2, so the float LLR differs only where x / s2 falls that close
-        // to a float rounding boundary
-        const double inv_s2 = 1.0 / s2;
+        // eq = y conj(h) / (|h|^2 + s2) through the hardware reciprocal (1 ulp) and
+        // 1 / s2_eq = 1 / max(s2 / |h|^2, 1e-8) = min(|h|^2 / s2, 1e8) through the
+        // launch's 1 / s2: no division per resource element.  The equalized point
+        // is within 2 float ulps and the scale within 2 double ulps of the
+        // divided values, so an LLR differs from the divided form by a few float
+        // ulps of its own magnitude (the sign only where d1 == d0 to that precision).
+        const float rden = __builtin_amdgcn_rcpf(h2 + nv);
+        const float er = tr * rden, ei = ti * rden;
+        const double inv_s2 = fmin(static_cast<double>(h2) * a.inv_noise_var_d, 1e8);
         // The Gray QAM table is a product of an in-phase and a quadrature level set:
         // point (gi << QB) | gq sits at (X[gi], Y[gq]), so its distance is
         // fl(DX[gi] + DY[gq]).  fl(a + b) is monotone in each operand, hence the
@@ -494,6 +582,7 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
 void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s)
 {
     if (frames <= 0) return;
+    if (!a.preamble_d) throw std::invalid_argument("sync: missing double-precision preamble");
     const int L = a.half_len;
     const int last = a.span - 2 * L;
     if (last < 0) throw std::invalid_argument("frame span shorter than the preamble");
@@ -501,7 +590,8 @@ void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s)
     const int wlen = std::min(last, limit + a.radius) + 2 * L;
     OfdmSyncArgs b = a;
     size_t lds = sizeof(float2) * static_cast<size_t>(wlen);
-    const size_t prefix_lds = lds + sizeof(double) * (2 * static_cast<size_t>(wlen - L + 1) + wlen + 1);
+    // the sliding search keeps the window energies of candidates 0 .. wlen - 2L
+    const size_t prefix_lds = lds + sizeof(double) * static_cast<size_t>(wlen - 2 * L + 1);
     b.window_in_lds = lds <= 56 * 1024 ? 1 : 0;
     b.prefix = prefix_lds <= 56 * 1024 ? 1 : 0;
     if (b.prefix)
