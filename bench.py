@@ -291,6 +291,18 @@ def calibrate_mix(a, sa, np, torch):
 # ------------------------------------------------------------------- served --
 SERVE_FRACS = (0.1, 0.25, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)
 SERVE_P99_SLO_MS = 2.0  # request-latency objective the served throughput is quoted at
+SERVE_DRIVES = 3        # Poisson drives per offered load (the median one is reported)
+
+
+def slo_point(loads):
+    """The highest offered load at which this and every lower load meet the p99
+    objective (the lowest load if even that one misses it)."""
+    best = loads[0]
+    for x in loads:
+        if x["p99_ms"] > SERVE_P99_SLO_MS:
+            break
+        best = x
+    return best
 
 
 def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
@@ -323,15 +335,23 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks):
         out["peak_jobs_per_launch"] = jobs_per_launch(s0, srv.stats())
         loads = []
         for frac in SERVE_FRACS:
+            # each offered load: SERVE_DRIVES drives with fresh Poisson arrivals, the
+            # median drive (by p99) reported, every drive's p99 beside it
             rate_jobs = frac * peak_mbps * 1e6 / (bits / n_jobs)  # jobs per second
-            arr = np.cumsum(rng.exponential(1e6 / rate_jobs, n_jobs))
+            runs = []
             s0 = srv.stats()
-            lat, wall = srv.drive(members, counts, ptrs, arr)
+            for _ in range(SERVE_DRIVES):
+                arr = np.cumsum(rng.exponential(1e6 / rate_jobs, n_jobs))
+                lat, wall = srv.drive(members, counts, ptrs, arr)
+                runs.append({"achieved_mbps": bits / wall / 1e6, "p50": float(np.percentile(lat, 50)),
+                             "p99": float(np.percentile(lat, 99)), "p999": float(np.percentile(lat, 99.9))})
+            med = sorted(runs, key=lambda r: r["p99"])[len(runs) // 2]
             loads.append({"offered_frac": frac, "offered_mbps": round(frac * peak_mbps, 2),
-                          "achieved_mbps": round(bits / wall / 1e6, 2),
-                          "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 4),
-                          "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 4),
-                          "p999_ms": round(float(np.percentile(lat, 99.9)) / 1e3, 4),
+                          "achieved_mbps": round(med["achieved_mbps"], 2),
+                          "p50_ms": round(med["p50"] / 1e3, 4),
+                          "p99_ms": round(med["p99"] / 1e3, 4),
+                          "p999_ms": round(med["p999"] / 1e3, 4),
+                          "p99_ms_drives": [round(r["p99"] / 1e3, 4) for r in runs],
                           "jobs_per_launch": jobs_per_launch(s0, srv.stats())})
         st = srv.stats()
         out["loads"] = loads
@@ -428,6 +448,25 @@ def leg_mixed(a, sa, np, torch, stream):
     md.decode(groups, use_graph=True, stream=sh)  # capture + first replay
     torch.cuda.synchronize()
     ms = timed(lambda: md.decode(groups, use_graph=True, stream=sh), a.mixed_reps, stream)
+    # Against what: each code decoded alone (its share of the batch, one launch), and
+    # the work-sum bound -- each code's codewords at its saturated per-codeword time
+    # (an 8x larger batch of the same LLRs), i.e. the makespan if every code ran at
+    # full-GPU throughput back to back with no tails.
+    alone, bound = [], []
+    for dec, g, (d_llr, d_info, d_ok, d_it) in zip(decs, groups, keep):
+        cnt = g.count
+        alone.append(timed(lambda: dec.decode_device(d_llr.data_ptr(), cnt, d_info=d_info.data_ptr(),
+                                                     d_ok=d_ok.data_ptr(), d_iters=d_it.data_ptr(), stream=sh),
+                           a.mixed_reps, stream))
+        big = d_llr.repeat(8, 1).contiguous()
+        bi = torch.empty((8 * cnt, dec.k), dtype=torch.uint8, device=dev)
+        dec.decode_device(big.data_ptr(), 8 * cnt, d_info=bi.data_ptr(), stream=sh)
+        bound.append(timed(lambda: dec.decode_device(big.data_ptr(), 8 * cnt, d_info=bi.data_ptr(), stream=sh),
+                           3, stream) / 8)
+        del big, bi
+    # the outputs of the last replay are read below: decode once more through the graph
+    md.decode(groups, use_graph=True, stream=sh)
+    torch.cuda.synchronize()
     per_code = {}
     fe_total = 0
     for (n, mb), info, (d_llr, d_info, d_ok, d_it), dec in zip(shapes, truth, keep, decs):
@@ -443,7 +482,9 @@ def leg_mixed(a, sa, np, torch, stream):
         d.close()
     return {"frames": F, "codes": len(shapes), "ms_per_batch": round(ms, 4),
             "value": round(info_bits_total / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
-            "fer": round(fe_total / F, 5), "graph": True, "per_code": per_code}
+            "fer": round(fe_total / F, 5), "graph": True,
+            "sum_alone_ms": round(sum(alone), 4), "work_sum_bound_ms": round(sum(bound), 4),
+            "makespan_over_bound": round(ms / sum(bound), 4), "per_code": per_code}
 
 
 # --------------------------------------------------------------------- ofdm --
@@ -909,17 +950,17 @@ def main() -> int:
     }
     out.update(extra)
     if "served" in extra:
-        # the served (128,64) decoder at the highest offered load whose p99 request
-        # latency meets the objective (the lowest load if none does)
-        loads = extra["served"]["ccsds128"]["loads"]
-        ok = [x for x in loads if x["p99_ms"] <= SERVE_P99_SLO_MS]
-        s = max(ok, key=lambda x: x["achieved_mbps"]) if ok else loads[0]
+        # the served (128,64) decoder at the highest offered load up to which every
+        # load's p99 request latency meets the objective
+        s = slo_point(extra["served"]["ccsds128"]["loads"])
         out["p50_ms"] = s["p50_ms"]
         out["p99_ms"] = s["p99_ms"]
         out["p999_ms"] = s["p999_ms"]
         out["served_mbps_at_p99"] = s["achieved_mbps"]
         out["served_p99_slo_ms"] = SERVE_P99_SLO_MS
         out["served_offered_frac"] = s["offered_frac"]
+        out["served_mix_mbps_at_p99"] = slo_point(extra["served"]["mix"]["loads"])["achieved_mbps"]
+        out["served_mix_offered_frac"] = slo_point(extra["served"]["mix"]["loads"])["offered_frac"]
     print(json.dumps(out))
     dec128.close()
     return 0

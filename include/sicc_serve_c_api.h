@@ -90,6 +90,25 @@ typedef struct sicc_server_stats_t {
 } sicc_server_stats_t;
 int sicc_server_stats(unsigned long long handle, sicc_server_stats_t* out);
 
+/* Scheduler log of recent launches, for latency breakdowns: steady-clock
+ * microseconds (CLOCK_MONOTONIC) at which the launch's oldest job was
+ * submitted, the scheduler took the launch's jobs, it finished enqueueing the
+ * launch's kernels, and it saw the launch complete; jobs, codewords and slot of
+ * the launch.  The last 65 536 launches are kept; reading returns up to `max`
+ * of them, oldest first, and forgets what it returned. */
+typedef struct sicc_server_launch_record_t {
+    unsigned long long t_first_submit_us;
+    unsigned long long t_taken_us;
+    unsigned long long t_issued_us;
+    unsigned long long t_done_us;
+    int jobs;
+    int codewords;
+    int slot;
+    int reserved;
+} sicc_server_launch_record_t;
+int sicc_server_launch_log(unsigned long long handle, sicc_server_launch_record_t* out, int max_records,
+                           int* out_count);
+
 #ifdef __cplusplus
 }
 #endif

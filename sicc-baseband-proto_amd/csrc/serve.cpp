@@ -118,6 +118,7 @@ struct Slot {
     GatherEntry* table = nullptr;  // pinned gather table of one launch
     PackGroup* groups = nullptr;   // pinned result-packing table, one group per member
     int table_cap = 0;
+    sicc_server_launch_record_t rec{};  // the launch in flight
 };
 
 class Server {
@@ -370,6 +371,17 @@ public:
 
     int device() const { return device_; }
 
+    int launch_log(sicc_server_launch_record_t* out, int max)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        int got = 0;
+        while (got < max && !log_.empty()) {
+            out[got++] = log_.front();
+            log_.pop_front();
+        }
+        return got;
+    }
+
 private:
     void loop()
     {
@@ -456,6 +468,9 @@ private:
         }
         lk.lock();
         for (Slot* s : done_) {
+            s->rec.t_done_us = t;
+            if (log_.size() >= kLogRecords) log_.pop_front();
+            log_.push_back(s->rec);
             for (auto& j : s->jobs) {
                 j->t_done = t;
                 if (j->off == SIZE_MAX)
@@ -546,6 +561,12 @@ private:
             }
             if (s.jobs.empty()) continue;
             const bool wait_admit = admit_pending_;
+            s.rec = sicc_server_launch_record_t{};
+            s.rec.t_first_submit_us = s.jobs.front()->t_submit;
+            s.rec.t_taken_us = now_us();
+            s.rec.jobs = static_cast<int>(s.jobs.size());
+            s.rec.codewords = total;
+            s.rec.slot = static_cast<int>(&s - slots_.data());
             lk.unlock();
             try {
                 issue(s, wait_admit);
@@ -553,6 +574,7 @@ private:
                 lk.lock();
                 throw;
             }
+            s.rec.t_issued_us = now_us();
             lk.lock();
             s.busy = true;
             ++st_.launches;
@@ -643,6 +665,8 @@ private:
     std::condition_variable cv_, cv_done_;
     std::deque<std::shared_ptr<Job>> arrivals_, resident_;
     std::deque<unsigned long long> completed_;
+    static constexpr size_t kLogRecords = 65536;
+    std::deque<sicc_server_launch_record_t> log_;  // completed launches, oldest first
     std::unordered_map<uint64_t, std::shared_ptr<Job>> known_;
     sicc_server_stats_t st_{};
     bool stop_ = false;
@@ -792,6 +816,16 @@ int sicc_server_drive(unsigned long long handle, int n_jobs, const int* member, 
         }
         s->clear_completed();
         if (wall_s) *wall_s = wall;
+        return CUDA_OSD_STATUS_OK;
+    });
+}
+
+int sicc_server_launch_log(unsigned long long handle, sicc_server_launch_record_t* out, int max_records,
+                           int* out_count)
+{
+    return guarded([&] {
+        if (!out_count || (max_records > 0 && !out)) throw std::invalid_argument("server_launch_log: null output");
+        *out_count = max_records > 0 ? slookup(handle)->launch_log(out, max_records) : 0;
         return CUDA_OSD_STATUS_OK;
     });
 }

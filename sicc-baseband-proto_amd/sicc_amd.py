@@ -437,6 +437,15 @@ class Server:
         check(self.lib.sicc_server_stats(self.handle, C.byref(s)))
         return s
 
+    def launch_log(self, max_records=65536):
+        """Recent launches (sicc_server_launch_log): structured array of steady-clock
+        microsecond times (first_submit, taken, issued, done), jobs, codewords, slot."""
+        rec = np.zeros(max_records, LAUNCH_RECORD_DTYPE)
+        got = C.c_int(0)
+        self.lib.sicc_server_launch_log.argtypes = [C.c_ulonglong, C.c_void_p, C.c_int, i32p]
+        check(self.lib.sicc_server_launch_log(self.handle, rec.ctypes.data, max_records, C.byref(got)))
+        return rec[:got.value]
+
     def close(self):
         if self.handle:
             check(self.lib.sicc_server_destroy(self.handle))
@@ -447,6 +456,11 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+LAUNCH_RECORD_DTYPE = np.dtype([("t_first_submit_us", np.uint64), ("t_taken_us", np.uint64),
+                                ("t_issued_us", np.uint64), ("t_done_us", np.uint64), ("jobs", np.int32),
+                                ("codewords", np.int32), ("slot", np.int32), ("reserved", np.int32)])
 
 
 class OfdmConfig(C.Structure):
