@@ -94,8 +94,10 @@ int sicc_server_stats(unsigned long long handle, sicc_server_stats_t* out);
  * microseconds (CLOCK_MONOTONIC) at which the launch's oldest job was
  * submitted, the scheduler took the launch's jobs, it finished enqueueing the
  * launch's kernels, and it saw the launch complete; jobs, codewords and slot of
- * the launch.  The last 65 536 launches are kept; reading returns up to `max`
- * of them, oldest first, and forgets what it returned. */
+ * the launch, and the number of streams its code groups ran on (the slot's own
+ * plus those of slots idle when it was issued).  The last 65 536 launches are
+ * kept; reading returns up to `max` of them, oldest first, and forgets what it
+ * returned. */
 typedef struct sicc_server_launch_record_t {
     unsigned long long t_first_submit_us;
     unsigned long long t_taken_us;
@@ -104,7 +106,7 @@ typedef struct sicc_server_launch_record_t {
     int jobs;
     int codewords;
     int slot;
-    int reserved;
+    int lanes;
 } sicc_server_launch_record_t;
 int sicc_server_launch_log(unsigned long long handle, sicc_server_launch_record_t* out, int max_records,
                            int* out_count);

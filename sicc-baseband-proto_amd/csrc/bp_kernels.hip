@@ -70,11 +70,31 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const GatherEntry* tab
     }
 }
 
+// Eight information bits (bytes of 0 / 1) to one byte, bit i from byte i.
+// The 8-byte load is aligned; the two 4-byte halves fold to bits 0 and 4 of
+// each byte, and one multiply gathers the four bytes' bits 0 / 4 into bits
+// 24..31 (every partial product lands on its own bit: no carries).
+__device__ __forceinline__ unsigned pack8_aligned(const uint8_t* p)
+{
+    const uint2 x = *reinterpret_cast<const uint2*>(p);
+    const unsigned t = (x.x & 0x01010101u) | ((x.y & 0x01010101u) << 4);
+    return (t * 0x01020408u) >> 24;
+}
+
+__device__ __forceinline__ unsigned pack_bits(const uint8_t* p, int nbits)
+{
+    unsigned b = 0;
+    for (int i = 0; i < nbits; ++i) b |= (p[i] & 1u) << i;
+    return b;
+}
+
 // Result packing for the server's copy back, all code groups of a launch in one
-// kernel: a block finds its group (few groups, block0 ascending), then one
-// wavefront packs 64 information bits of a row (ballot -> 8 bytes, bit i of a
-// row at byte i/8, bit i%8), or, in the group's trailing blocks, one thread
-// copies one byte of the per-codeword flags behind the packed rows.
+// kernel.  A group's output is one byte array: its rows' information bits,
+// bit i of a row at byte i / 8, bit i % 8 (kb = ceil(k / 8) bytes a row), then
+// its flag bytes.  Each thread builds four consecutive output bytes and writes
+// them with one 4-byte store (group outputs start 4-byte aligned), so every
+// wavefront writes 256 contiguous bytes of the pinned host results instead of
+// byte-wide scattered stores across the bus.
 __global__ __launch_bounds__(256) void pack_groups_kernel(const PackGroup* __restrict__ groups, int count)
 {
     // the table may sit in host memory: fetch the group starts in one parallel
@@ -91,22 +111,40 @@ __global__ __launch_bounds__(256) void pack_groups_kernel(const PackGroup* __res
     }
     const PackGroup g = groups[gi];
     const int kb = (g.k + 7) >> 3;
-    const int b = static_cast<int>(blockIdx.x) - g.block0;
-    if (b >= g.pack_blocks) {
-        const int i = (b - g.pack_blocks) * 256 + threadIdx.x;
-        if (i < g.flag_bytes) g.out[static_cast<size_t>(g.rows) * kb + i] = g.flags[i];
-        return;
+    const size_t packed = static_cast<size_t>(g.rows) * kb;
+    const size_t total = packed + static_cast<size_t>(g.flag_bytes);
+    const size_t q0 = (static_cast<size_t>(static_cast<int>(blockIdx.x) - g.block0) * 256 + threadIdx.x) * 4;
+    if (q0 >= total) return;
+    const bool vec = (g.k & 7) == 0 && (reinterpret_cast<uintptr_t>(g.bits) & 7u) == 0;
+    size_t r = 0;
+    int j = 0;
+    if (q0 < packed) {
+        r = q0 / static_cast<size_t>(kb);
+        j = static_cast<int>(q0 - r * kb);
     }
-    const int lane = threadIdx.x & 63;
-    const int chunks = (g.k + 63) >> 6;
-    const long w = static_cast<long>(b) * 4 + (threadIdx.x >> 6);
-    if (w >= static_cast<long>(g.rows) * chunks) return;  // whole wavefront
-    const long r = w / chunks;
-    const int c = static_cast<int>(w - r * chunks);
-    const int i = c * 64 + lane;
-    const unsigned long long m = __ballot(i < g.k && g.bits[r * g.k + i] != 0);
-    const int nb = min(8, kb - c * 8);
-    if (lane < nb) g.out[r * kb + c * 8 + lane] = static_cast<uint8_t>(m >> (8 * lane));
+    unsigned word = 0;
+    const int nb = total - q0 < 4 ? static_cast<int>(total - q0) : 4;
+    for (int b = 0; b < nb; ++b) {
+        const size_t q = q0 + b;
+        unsigned v;
+        if (q < packed) {
+            const uint8_t* p = g.bits + r * g.k + 8 * j;
+            v = vec ? pack8_aligned(p) : pack_bits(p, min(8, g.k - 8 * j));
+            if (++j == kb) {
+                j = 0;
+                ++r;
+            }
+        } else {
+            v = g.flags[q - packed];
+        }
+        word |= v << (8 * b);
+    }
+    uint8_t* dst = g.out + q0;
+    if (nb == 4) {
+        *reinterpret_cast<unsigned*>(dst) = word;
+    } else {
+        for (int b = 0; b < nb; ++b) dst[b] = static_cast<uint8_t>(word >> (8 * b));
+    }
 }
 
 // XOR of N words with three-input XORs (v_bitop3_b32, truth table 0x96):
@@ -136,8 +174,10 @@ __device__ __forceinline__ float col_sum(float a, const float* r)
     return a;
 }
 
+// One team's decode loop: team `gteam` of `nteams` takes codewords gteam,
+// gteam + nteams, ... (or chunks from io.work_ctr).
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
-__device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad)
+__device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad, int gteam, int nteams)
 {
     extern __shared__ float lds[];
     constexpr int T = 64 * TEAM;     // threads per team
@@ -180,9 +220,6 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
         red[0] = red[1] = 0;
     }
 
-    const int teams_per_block = blockDim.x / T;
-    const int gteam = blockIdx.x * teams_per_block + team;
-    const int nteams = gridDim.x * teams_per_block;
     const float alpha = p.alpha;
     const float off = p.offset;
     const float damp = p.damping;
@@ -495,11 +532,46 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
     }
 }
 
+// Team index and team count of a one-batch launch (teams of T threads).
+template <int T>
+__device__ __forceinline__ int grid_team() { return static_cast<int>(blockIdx.x * (blockDim.x / T) + threadIdx.x / T); }
+template <int T>
+__device__ __forceinline__ int grid_teams() { return static_cast<int>(gridDim.x * (blockDim.x / T)); }
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
 __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
                                                                               int e_pad)
 {
-    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(code, p, io, e_pad);
+    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(code, p, io, e_pad, grid_team<64 * TEAM>(), grid_teams<64 * TEAM>());
+}
+
+// Several batches, of codes that share one instance of the decode loop, in one
+// launch: segment i owns blocks [block0, block0 of segment i + 1) and its
+// `teams` teams stride over its own batch with its own code, parameters and
+// outputs.  The per-codeword arithmetic is the one-batch kernel's.
+struct BpSegment {
+    BpCodeDev code;
+    BpParams p;
+    BpIo io;
+    int e_pad;
+    int block0;
+    int teams;
+};
+constexpr int kBpSegmentsPerLaunch = 8;
+struct BpSegmentTable {
+    BpSegment seg[kBpSegmentsPerLaunch];
+    int count;
+};
+
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_multi_kernel(BpSegmentTable t)
+{
+    constexpr int T = 64 * TEAM;
+    int si = 0;
+    while (si + 1 < t.count && static_cast<int>(blockIdx.x) >= t.seg[si + 1].block0) ++si;
+    const BpSegment& g = t.seg[si];
+    const int gteam = static_cast<int>((blockIdx.x - g.block0) * (blockDim.x / T) + threadIdx.x / T);
+    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(g.code, g.p, g.io, g.e_pad, gteam, g.teams);
 }
 
 // The single-wave shape of codes up to 128 variables and row degree 8 without
@@ -535,14 +607,14 @@ __global__ __launch_bounds__(256) void bp_flood_kernel_lean(BpCodeDev code, BpPa
     io.osd_llr = l.osd_llr;
     io.osd_index = l.osd_index;
     io.osd_count = l.osd_count;
-    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad);
+    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad, grid_team<64>(), grid_teams<64>());
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bp_flood_kernel_w8(BpCodeDev code,
                                                                                                   BpParams p,
                                                                                                   BpIo io, int e_pad)
 {
-    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad);
+    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad, grid_team<64>(), grid_teams<64>());
 }
 
 template <int MPL, int NPL, int MAXDC, int TEAM>
@@ -584,6 +656,76 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("bp_flood_kernel launch: ") + hipGetErrorString(e));
+}
+
+// Up to kBpSegmentsPerLaunch batches of one instance in one launch.  Each
+// batch gets the blocks its codewords (capped by its max_waves) need; when
+// they exceed what the GPU holds at once, the resident blocks are shared in
+// proportion, so the batches' teams all start together and each strides over
+// its own batch.
+template <int MPL, int NPL, int MAXDC, int TEAM>
+void launch_multi_variant(const BpBatch* b, int count, hipStream_t s)
+{
+    constexpr int T = 64 * TEAM;
+    const bool damp = b[0].p.damping != 0.0f;
+    int e_pad[kBpSegmentsPerLaunch];
+    int e_max = 0;
+    for (int i = 0; i < count; ++i) {
+        e_pad[i] = (b[i].code.E + 1 + 63) & ~63;
+        e_max = e_pad[i] > e_max ? e_pad[i] : e_max;
+    }
+    const size_t team_bytes = sizeof(float) * static_cast<size_t>(8 + T * NPL + 4 + e_max);
+    const int tpb_req = b[0].tune.teams_per_block;
+    int tpb = TEAM == 1 ? (tpb_req >= 4 ? 4 : (tpb_req >= 2 ? 2 : 1)) : 1;
+    while (tpb > 1 && team_bytes * tpb > 64 * 1024) --tpb;
+    if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
+    const size_t lds = team_bytes * tpb;
+    auto kern = damp ? bp_flood_multi_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_multi_kernel<MPL, NPL, MAXDC, false, TEAM>;
+    const long resident =
+        static_cast<long>(resident_blocks_per_cu(reinterpret_cast<const void*>(kern), T * tpb, lds)) * device_cu_count();
+    long need[kBpSegmentsPerLaunch];
+    long total_need = 0;
+    for (int i = 0; i < count; ++i) {
+        const int mw = b[i].max_waves / TEAM;
+        long teams = b[i].io.num_cw < mw ? b[i].io.num_cw : mw;
+        if (teams < 1) teams = 1;
+        need[i] = (teams + tpb - 1) / tpb;
+        total_need += need[i];
+    }
+    BpSegmentTable t{};
+    t.count = count;
+    long blocks = 0;
+    for (int i = 0; i < count; ++i) {
+        long nb = need[i];
+        if (total_need > resident && resident > 0) nb = need[i] * resident / total_need;
+        if (nb < 1) nb = 1;
+        BpSegment& g = t.seg[i];
+        g.code = b[i].code;
+        g.p = b[i].p;
+        g.io = b[i].io;
+        g.io.work_ctr = nullptr;
+        g.e_pad = e_pad[i];
+        g.block0 = static_cast<int>(blocks);
+        g.teams = static_cast<int>(nb * tpb);
+        blocks += nb;
+    }
+    if (blocks > 0x7fffffffL) throw std::invalid_argument("fused BP launch too large");
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(blocks)), dim3(T * tpb), lds, s, t);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("bp_flood_multi_kernel launch: ") + hipGetErrorString(e));
+}
+
+template <int MPL, int NPL, int MAXDC>
+void launch_multi_team(int team, const BpBatch* b, int count, hipStream_t s)
+{
+    switch (team) {
+        case 1: return launch_multi_variant<MPL, NPL, MAXDC, 1>(b, count, s);
+        case 2: return launch_multi_variant<MPL, NPL, MAXDC, 2>(b, count, s);
+        case 4: return launch_multi_variant<MPL, NPL, MAXDC, 4>(b, count, s);
+        case 8: return launch_multi_variant<MPL, NPL, MAXDC, 8>(b, count, s);
+        case 16: return launch_multi_variant<MPL, NPL, MAXDC, 16>(b, count, s);
+        default: throw std::invalid_argument("BP team size must be 1, 2, 4, 8 or 16 waves");
+    }
 }
 
 template <int MPL, int NPL, int MAXDC>
@@ -631,10 +773,11 @@ void launch_pack_groups(PackGroup* groups, int count, hipStream_t s)
     for (int i = 0; i < count; ++i) {
         PackGroup& g = groups[i];
         if (g.rows <= 0 || g.k <= 0 || g.flag_bytes < 0) throw std::invalid_argument("pack_groups: bad shape");
-        const long waves = static_cast<long>(g.rows) * ((g.k + 63) / 64);
+        if (reinterpret_cast<uintptr_t>(g.out) & 3u) throw std::invalid_argument("pack_groups: output not 4-byte aligned");
+        const long bytes = static_cast<long>(g.rows) * ((g.k + 7) / 8) + g.flag_bytes;
         g.block0 = static_cast<int>(blocks);
-        g.pack_blocks = static_cast<int>((waves + 3) / 4);
-        blocks += g.pack_blocks + (g.flag_bytes + 255) / 256;
+        g.pack_blocks = static_cast<int>((bytes + 1023) / 1024);  // 4 bytes a thread
+        blocks += g.pack_blocks;
         if (blocks > 0x7fffffffL) throw std::invalid_argument("pack_groups: launch too large");
     }
     hipLaunchKernelGGL(pack_groups_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, groups, count);
@@ -668,6 +811,48 @@ int bp_default_team(const BpCodeDev& c)
 }
 
 bool bp_shape_supported(const BpCodeDev& c) { return bp_default_team(c) != 0; }
+
+int bp_flood_instance(const BpCodeDev& c, const BpParams& p, const BpTune& tune)
+{
+    const int team = tune.team > 0 ? tune.team : bp_default_team(c);
+    if (!bp_team_supported(c, team) || p.max_iter < 1) return -1;
+    int mpl = 0, npl = 0;
+    per_thread(c, team, &mpl, &npl);
+    const bool small = mpl <= 1 && npl <= 2;
+    const int tpb = team == 1 ? (tune.teams_per_block >= 4 ? 4 : (tune.teams_per_block >= 2 ? 2 : 1)) : 1;
+    return team << 8 | tpb << 4 | (small ? 4 : 0) | (c.max_dc <= 8 ? 2 : 0) | (p.damping != 0.0f ? 1 : 0);
+}
+
+void launch_bp_flood_multi(const BpBatch* batches, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    const int key = bp_flood_instance(batches[0].code, batches[0].p, batches[0].tune);
+    if (key < 0) throw std::invalid_argument("code shape not supported by the BP kernel");
+    for (int i = 0; i < count; ++i) {
+        if (bp_flood_instance(batches[i].code, batches[i].p, batches[i].tune) != key)
+            throw std::invalid_argument("fused BP batches must share one kernel instance");
+        if (batches[i].io.num_cw <= 0 || !batches[i].io.llr) throw std::invalid_argument("fused BP batch is empty");
+        if (batches[i].io.work_ctr) throw std::invalid_argument("fused BP batches use static striding");
+    }
+    const int team = key >> 8;
+    const bool small = (key & 4) != 0;
+    const bool dc8 = (key & 2) != 0;
+    for (int c0 = 0; c0 < count; c0 += kBpSegmentsPerLaunch) {
+        const int c = count - c0 < kBpSegmentsPerLaunch ? count - c0 : kBpSegmentsPerLaunch;
+        const BpBatch* b = batches + c0;
+        if (dc8) {
+            if (small)
+                launch_multi_team<1, 2, 8>(team, b, c, s);
+            else
+                launch_multi_team<2, 4, 8>(team, b, c, s);
+        } else {
+            if (small)
+                launch_multi_team<1, 2, 16>(team, b, c, s);
+            else
+                launch_multi_team<2, 4, 16>(team, b, c, s);
+        }
+    }
+}
 
 void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves, hipStream_t s,
                      const BpTune& tune)

@@ -331,28 +331,67 @@ std::vector<unsigned char> group_params(const MultiDec& md, const sicc_group_t* 
     return k;
 }
 
-// Enqueue every group on its own lane stream, forked from and joined back to
-// `s`; group i decodes with scratch[i].
+// Enqueue the groups on the lane streams, forked from and joined back to `s`;
+// group i decodes with scratch[i].  BP-only groups whose batches share one
+// instance of the flooding kernel decode in one fused launch (their slowest
+// codewords overlap instead of queueing behind each other); every other group
+// is a launch of its own.  Launches are dealt round-robin over the lanes in
+// the order of their first group.
 void enqueue_groups(MultiDec& md, const sicc_group_t* groups, int n, hipStream_t s,
                     std::vector<std::unique_ptr<DecodeScratch>>& scratch)
 {
     while (static_cast<int>(scratch.size()) < n) scratch.push_back(std::make_unique<DecodeScratch>());
-    SICC_HIP(hipEventRecord(md.fork, s));
+    std::vector<DecodeOut> outs(n);
+    std::vector<BpBatch> batch(n);
+    std::vector<int> key(n, -1);
     for (int i = 0; i < n; ++i) {
         const sicc_group_t& g = groups[i];
         Decoder& d = *md.members[g.member];
-        hipStream_t ls = md.lanes[i % md.lanes.size()];
-        SICC_HIP(hipStreamWaitEvent(ls, md.fork, 0));
-        DecodeOut out;
+        DecodeOut& out = outs[i];
         out.info = g.d_info;
         out.bp_ok = g.d_bp_ok;
         out.iterations = g.d_iterations;
         out.used_osd = g.d_used_osd;
-        d.eng->decode(g.d_llr, g.count, d.bp, d.kind == Kind::Stream ? d.osd_order : -1, d.osd, out, ls,
-                      scratch[i].get());
-        SICC_HIP(hipEventRecord(md.joins[i % md.joins.size()], ls));
+        const bool bp_only = d.kind != Kind::Stream || d.osd_order < 0;
+        if (bp_only && d.eng->bp_batch(g.d_llr, g.count, d.bp, out, &batch[i]))
+            key[i] = bp_flood_instance(batch[i].code, batch[i].p, batch[i].tune);
     }
-    const int used = std::min<int>(n, static_cast<int>(md.lanes.size()));
+    // launches: lists of group indices (a fused list shares one key)
+    std::vector<std::vector<int>> launches;
+    std::vector<int> launch_of_key;  // parallel to keys_seen
+    std::vector<int> keys_seen;
+    for (int i = 0; i < n; ++i) {
+        if (key[i] >= 0) {
+            const auto it = std::find(keys_seen.begin(), keys_seen.end(), key[i]);
+            if (it != keys_seen.end()) {
+                launches[launch_of_key[it - keys_seen.begin()]].push_back(i);
+                continue;
+            }
+            keys_seen.push_back(key[i]);
+            launch_of_key.push_back(static_cast<int>(launches.size()));
+        }
+        launches.push_back({i});
+    }
+    SICC_HIP(hipEventRecord(md.fork, s));
+    const int nl = static_cast<int>(launches.size());
+    for (int li = 0; li < nl; ++li) {
+        const std::vector<int>& L = launches[li];
+        hipStream_t ls = md.lanes[li % md.lanes.size()];
+        SICC_HIP(hipStreamWaitEvent(ls, md.fork, 0));
+        if (L.size() > 1) {
+            std::vector<BpBatch> fb;
+            for (int i : L) fb.push_back(batch[i]);
+            launch_bp_flood_multi(fb.data(), static_cast<int>(fb.size()), ls);
+        } else {
+            const int i = L[0];
+            const sicc_group_t& g = groups[i];
+            Decoder& d = *md.members[g.member];
+            d.eng->decode(g.d_llr, g.count, d.bp, d.kind == Kind::Stream ? d.osd_order : -1, d.osd, outs[i], ls,
+                          scratch[i].get());
+        }
+        SICC_HIP(hipEventRecord(md.joins[li % md.joins.size()], ls));
+    }
+    const int used = std::min<int>(nl, static_cast<int>(md.lanes.size()));
     for (int i = 0; i < used; ++i) SICC_HIP(hipStreamWaitEvent(s, md.joins[i], 0));
 }
 

@@ -350,6 +350,28 @@ void Engine::decode_short(const float* d_llr, int num_cw, const BpParams& bp, in
     }
 }
 
+bool Engine::bp_batch(const float* d_llr, int num_cw, const BpParams& bp, const DecodeOut& out, BpBatch* b) const
+{
+    if (!short_ok_ || uses_long_kernel(bp) || tune_.steal_chunk > 0 || num_cw <= 0 || !d_llr) return false;
+    if (bp_flood_instance(bpdev_, bp, tune_) < 0) return false;
+    b->code = bpdev_;
+    b->p = bp;
+    b->tune = tune_;
+    b->max_waves = tune_.max_waves > 0 ? tune_.max_waves : max_waves_;
+    BpIo io;
+    io.llr = d_llr;
+    io.num_cw = num_cw;
+    io.code_bits = out.code;
+    io.info_bits = out.info;
+    io.app = out.app;
+    io.success = out.bp_ok;
+    io.iterations = out.iterations;
+    io.used_osd = out.used_osd;
+    io.distance = out.distance;
+    b->io = io;
+    return true;
+}
+
 void Engine::osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s)
 {
     if (num_cw <= 0) return;

@@ -73,6 +73,11 @@ public:
     // long-code kernel or OSD needs more than v.cap hand-off slots.
     void decode_into(const float* d_llr, int num_cw, const BpParams& bp, int osd_order, const OsdParams& osd,
                      const DecodeOut& out, hipStream_t s, const ScratchView& v);
+    // The flooding-kernel batch decode(d_llr, num_cw, bp, -1, ...) runs, for a
+    // launch that decodes it together with other codes' batches
+    // (launch_bp_flood_multi; no kernel timing).  False when that decode cannot
+    // share a launch: the long-code kernel, or work stealing.
+    bool bp_batch(const float* d_llr, int num_cw, const BpParams& bp, const DecodeOut& out, BpBatch* b) const;
     // OSD on every codeword (no BP).
     void osd_only(const float* d_llr, int num_cw, const OsdParams& osd, const DecodeOut& out, hipStream_t s);
 

@@ -97,9 +97,9 @@ struct GatherEntry {
 void launch_gather_rows(const GatherEntry* table, int entries, hipStream_t s);
 // Result packing of one code group: `rows` rows of k 0/1 bytes at `bits` become
 // ceil(k/8) bytes each at `out` (bit i at byte i/8, bit i%8), followed by
-// `flag_bytes` bytes copied from `flags`.  Blocks [block0, block0 + pack_blocks
-// + ceil(flag_bytes / 256)) of the launch serve the group (pack_blocks =
-// ceil(rows ceil(k/64) / 4)).
+// `flag_bytes` bytes copied from `flags`; `out` must be 4-byte aligned.
+// Blocks [block0, block0 + pack_blocks) of the launch serve the group, 1 KiB of
+// its output each (pack_blocks = ceil((rows ceil(k/8) + flag_bytes) / 1024)).
 struct PackGroup {
     const uint8_t* bits;
     const uint8_t* flags;
@@ -137,6 +137,21 @@ bool bp_team_supported(const BpCodeDev& c, int team);
 int bp_default_team(const BpCodeDev& c);
 void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, int max_waves,
                      hipStream_t s, const BpTune& tune = BpTune());
+// One batch of the flooding kernel, for launches that decode the batches of
+// several codes together (work stealing off: io.work_ctr must be null).
+struct BpBatch {
+    BpCodeDev code;
+    BpParams p;
+    BpIo io;
+    BpTune tune;
+    int max_waves = 0;
+};
+// Key of the kernel instance (launch shape, row-degree class, damping) a batch
+// runs on; batches with equal keys can share a launch.  -1: no instance fits.
+int bp_flood_instance(const BpCodeDev& code, const BpParams& p, const BpTune& tune);
+// Decodes batches of one instance key, up to eight per launch, each with its
+// own code, parameters and outputs; results equal launch_bp_flood's per batch.
+void launch_bp_flood_multi(const BpBatch* batches, int count, hipStream_t s);
 // Long-code BP (flooding or layered), one workgroup of 16 wavefronts per
 // codeword (8 for row degrees above 16).
 // Rows are processed in groups (layered: colours of the greedy row colouring;

@@ -108,6 +108,8 @@ struct Job {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t gathered = nullptr;        // the launch's packed LLRs are in place
+    std::vector<hipEvent_t> lane_done;    // per lane stream: the launch's last group there
     bool busy = false;
     std::vector<std::shared_ptr<Job>> jobs;
     std::vector<DevBuf<float>> pack_llr;    // per member, max_cw rows
@@ -160,10 +162,12 @@ private:
         for (auto& s : slots_) {
             SICC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
             SICC_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+            SICC_HIP(hipEventCreateWithFlags(&s.gathered, hipEventDisableTiming));
             s.pack_llr.resize(members_.size());
             s.pack_out.resize(members_.size());
             s.scratch.resize(members_.size());
-            s.h_res_bytes = static_cast<size_t>(max_cw_) * (kbmax + 2);
+            // every group's block starts 4-byte aligned (the packing kernel's stores)
+            s.h_res_bytes = static_cast<size_t>(max_cw_) * (kbmax + 2) + 4 * members_.size();
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_res), s.h_res_bytes, hipHostMallocDefault));
             s.table_cap = max_cw_;  // at most one entry per codeword
             SICC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.table), sizeof(GatherEntry) * s.table_cap,
@@ -182,6 +186,11 @@ private:
                 d.eng->reserve(osd >= 0 ? max_cw_ : 0, &d.bp, &s.scratch[m]);
             }
         }
+        // every slot's stream is also a lane for the code groups of any launch
+        for (auto& s : slots_) {
+            s.lane_done.resize(slots_.size(), nullptr);
+            for (auto& e : s.lane_done) SICC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
     }
 
     // HIP handles and pinned buffers (device buffers free themselves).
@@ -193,6 +202,9 @@ private:
             if (s.table) (void)hipHostFree(s.table);
             if (s.groups) (void)hipHostFree(s.groups);
             if (s.done) (void)hipEventDestroy(s.done);
+            if (s.gathered) (void)hipEventDestroy(s.gathered);
+            for (auto e : s.lane_done)
+                if (e) (void)hipEventDestroy(e);
             if (s.stream) (void)hipStreamDestroy(s.stream);
         }
         if (admit_stream_) {
@@ -618,9 +630,27 @@ private:
                 }
             }
             launch_gather_rows(s.table, tbl, s.stream);
+            // Decode units: the code groups whose BP batches share one instance of
+            // the flooding kernel form one fused launch (their slowest codewords
+            // then overlap instead of running one kernel after another); every
+            // other group (OSD, long-code kernel, work stealing) is a unit of its
+            // own.  With several units they are dealt over the slot's own stream
+            // and the streams of the slots that are idle (one hardware queue
+            // each), largest first onto the least loaded lane (edges x
+            // codewords); with every slot busy the launch stays on its own stream,
+            // the queues being full anyway.  Each lane waits for the gather; the
+            // slot's stream joins every lane before packing the results.
+            const int nslots = static_cast<int>(slots_.size());
+            const int self = static_cast<int>(&s - slots_.data());
+            const size_t nm = members_.size();
+            g_out_.resize(nm);
+            g_llr_.resize(nm);
+            g_cw_.resize(nm);
+            g_batch_.resize(nm);
             size_t res = 0;
             int ng = 0;
-            for (size_t m = 0; m < by.size(); ++m) {
+            int nu = 0;
+            for (size_t m = 0; m < nm; ++m) {
                 if (by[m].empty()) continue;
                 Decoder& d = *members_[m];
                 const LdpcCode& c = d.eng->code();
@@ -632,7 +662,26 @@ private:
                 out.info = o;
                 out.bp_ok = o + static_cast<size_t>(cw) * c.k;
                 out.used_osd = o + static_cast<size_t>(cw) * (c.k + 1);
-                d.eng->decode(llr, cw, d.bp, osd, d.osd, out, s.stream, &s.scratch[m]);
+                g_out_[m] = out;
+                g_llr_[m] = llr;
+                g_cw_[m] = cw;
+                int key = -1;
+                if (osd < 0 && d.eng->bp_batch(llr, cw, d.bp, out, &g_batch_[m]))
+                    key = bp_flood_instance(g_batch_[m].code, g_batch_[m].p, g_batch_[m].tune);
+                int u = 0;
+                if (key >= 0)
+                    while (u < nu && units_[u].key != key) ++u;
+                else
+                    u = nu;
+                if (u == nu) {
+                    if (static_cast<int>(units_.size()) <= nu) units_.emplace_back();
+                    units_[nu].key = key;
+                    units_[nu].cost = 0;
+                    units_[nu].members.clear();
+                    ++nu;
+                }
+                units_[u].members.push_back(static_cast<int>(m));
+                units_[u].cost += std::max(1LL, static_cast<long long>(cw) * c.edges());
                 // results leave bit-packed (8x less to move and to read on the host),
                 // ok / used flags behind them, written by the packing kernel straight
                 // into the slot's pinned results: no copy-engine transfer, whose D2H
@@ -642,8 +691,51 @@ private:
                     j->grp_base = res;
                     j->grp_cw = cw;
                 }
-                res += static_cast<size_t>(cw) * ((c.k + 7) / 8 + 2);
+                res = (res + static_cast<size_t>(cw) * ((c.k + 7) / 8 + 2) + 3) & ~static_cast<size_t>(3);
             }
+            lanes_.assign(1, self);
+            for (int l = 0; l < nslots; ++l)
+                if (l != self && !slots_[l].busy) lanes_.push_back(l);
+            order_.resize(nu);
+            for (int u = 0; u < nu; ++u) order_[u] = u;
+            const bool spread = nu > 1 && lanes_.size() > 1;
+            if (spread) {
+                std::stable_sort(order_.begin(), order_.end(),
+                                 [&](int x, int y) { return units_[x].cost > units_[y].cost; });
+                SICC_HIP(hipEventRecord(s.gathered, s.stream));
+            }
+            lane_load_.assign(lanes_.size(), 0);
+            for (int ui : order_) {
+                const Unit& un = units_[ui];
+                size_t li = 0;
+                if (spread)
+                    for (size_t j = 1; j < lanes_.size(); ++j)
+                        if (lane_load_[j] < lane_load_[li]) li = j;
+                const int lane = lanes_[li];
+                hipStream_t ls = slots_[lane].stream;
+                if (lane != self && lane_load_[li] == 0) SICC_HIP(hipStreamWaitEvent(ls, s.gathered, 0));
+                lane_load_[li] += un.cost;
+                if (un.key >= 0 && un.members.size() > 1) {
+                    fused_.clear();
+                    for (int m : un.members) fused_.push_back(g_batch_[m]);
+                    launch_bp_flood_multi(fused_.data(), static_cast<int>(fused_.size()), ls);
+                } else {
+                    for (int m : un.members) {
+                        Decoder& d = *members_[m];
+                        const int osd = d.kind == Kind::Stream ? d.osd_order : -1;
+                        d.eng->decode(g_llr_[m], g_cw_[m], d.bp, osd, d.osd, g_out_[m], ls, &s.scratch[m]);
+                    }
+                }
+            }
+            for (size_t j = 1; j < lanes_.size(); ++j)
+                if (lane_load_[j]) {
+                    const int l = lanes_[j];
+                    SICC_HIP(hipEventRecord(s.lane_done[l], slots_[l].stream));
+                    SICC_HIP(hipStreamWaitEvent(s.stream, s.lane_done[l], 0));
+                }
+            s.rec.lanes = spread ? static_cast<int>(std::count_if(lane_load_.begin(), lane_load_.end(),
+                                                                  [](long long x) { return x > 0; }))
+                                 : 1;
             launch_pack_groups(s.groups, ng, s.stream);
             SICC_HIP(hipEventRecord(s.done, s.stream));
         }
@@ -661,6 +753,21 @@ private:
     std::vector<Slot> slots_;
     std::vector<Slot*> done_;                 // harvest's completed slots
     std::vector<std::vector<Job*>> by_;       // issue's per-member job lists
+    // issue's decode units: fused BP launches (key = kernel instance) or single groups
+    struct Unit {
+        int key = -1;
+        long long cost = 0;
+        std::vector<int> members;
+    };
+    std::vector<Unit> units_;
+    std::vector<int> order_;                  // issue's units, largest first
+    std::vector<int> lanes_;                  // issue's lanes: its own slot, then idle slots
+    std::vector<long long> lane_load_;        // issue's work dealt to each lane
+    std::vector<DecodeOut> g_out_;            // issue's per-member outputs
+    std::vector<const float*> g_llr_;         // issue's per-member LLR rows
+    std::vector<int> g_cw_;                   // issue's per-member codeword counts
+    std::vector<BpBatch> g_batch_;            // issue's per-member BP batches
+    std::vector<BpBatch> fused_;              // issue's batches of one fused launch
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
     std::deque<std::shared_ptr<Job>> arrivals_, resident_;

@@ -460,7 +460,7 @@ class Server:
 
 LAUNCH_RECORD_DTYPE = np.dtype([("t_first_submit_us", np.uint64), ("t_taken_us", np.uint64),
                                 ("t_issued_us", np.uint64), ("t_done_us", np.uint64), ("jobs", np.int32),
-                                ("codewords", np.int32), ("slot", np.int32), ("reserved", np.int32)])
+                                ("codewords", np.int32), ("slot", np.int32), ("lanes", np.int32)])
 
 
 class OfdmConfig(C.Structure):

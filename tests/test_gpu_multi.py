@@ -193,3 +193,66 @@ def test_two_groups_of_one_member_and_parameter_changes():
                 md.decode(groups(), use_graph=use_graph)
                 torch.cuda.synchronize()
                 check(2)
+
+
+def test_fused_launch_per_code_parameters():
+    """Groups of codes that share one BP kernel instance decode in one fused
+    launch with their own code, iteration cap, normalization and offset; a pair
+    with damping shares another instance.  Batches far larger than the resident
+    teams (each team strides over its own code's batch) equal every decoder's
+    own decode exactly, eager and in a graph, and the oracle on a sample."""
+    import torch
+
+    print("[case] fused BP launch, per-code parameters", flush=True)
+    rng = np.random.default_rng(77)
+    params = [dict(max_iter=30, alpha=0.75), dict(max_iter=12, alpha=0.8), dict(max_iter=40, alpha=0.7, offset=0.1),
+              dict(max_iter=25, alpha=0.75, damping=0.25), dict(max_iter=20, alpha=0.8, damping=0.5)]
+    counts = [30000, 20000, 9000, 12000, 7000]
+    decs, codes, data = [], [], []
+    for i, (pr, cnt) in enumerate(zip(params, counts)):
+        m, rp, ci = sa.qc_code(256, 4, 3, seed=900 + i)
+        code = oracle.Code(256, m, rp, ci, front_info=True)
+        info = rng.integers(0, 2, (cnt, code.k), dtype=np.uint8)
+        llr = oracle.awgn_llr(code.encode(info), 2.0, code.k / 256, rng)
+        decs.append(sa.Decoder.bp_csr(256, m, rp, ci, schedule=0, **pr))
+        codes.append(code)
+        data.append({"llr": llr, "d_llr": torch.from_numpy(llr).cuda(),
+                     "d_info": torch.zeros((cnt, code.k), dtype=torch.uint8, device="cuda"),
+                     "d_ok": torch.zeros(cnt, dtype=torch.uint8, device="cuda"),
+                     "d_it": torch.zeros(cnt, dtype=torch.int32, device="cuda")})
+    alone = []
+    for dec, d in zip(decs, data):
+        cnt = d["llr"].shape[0]
+        a_info = torch.zeros((cnt, dec.k), dtype=torch.uint8, device="cuda")
+        a_ok = torch.zeros(cnt, dtype=torch.uint8, device="cuda")
+        a_it = torch.zeros(cnt, dtype=torch.int32, device="cuda")
+        dec.decode_device(d["d_llr"].data_ptr(), cnt, d_info=C_ptr(a_info), d_ok=C_ptr(a_ok), d_iters=C_ptr(a_it))
+        torch.cuda.synchronize()
+        alone.append((a_info.cpu().numpy(), a_ok.cpu().numpy(), a_it.cpu().numpy()))
+    with sa.MultiDecoder(decs) as md:
+        for use_graph in (False, True, True):
+            for d in data:
+                d["d_info"].zero_()
+                d["d_ok"].fill_(7)
+                d["d_it"].fill_(-1)
+            torch.cuda.synchronize()
+            md.decode(_groups(data), use_graph=use_graph)
+            torch.cuda.synchronize()
+            for j, (d, (info, ok, it)) in enumerate(zip(data, alone)):
+                assert np.array_equal(d["d_info"].cpu().numpy(), info), (j, use_graph)
+                assert np.array_equal(d["d_ok"].cpu().numpy(), ok), (j, use_graph)
+                assert np.array_equal(d["d_it"].cpu().numpy(), it), (j, use_graph)
+    for code, pr, d, (info, ok, it) in zip(codes, params, data, alone):
+        assert 0 < ok.sum() < ok.size
+        for i in range(0, d["llr"].shape[0], 211):
+            o = code.nms(d["llr"][i], **pr)
+            assert ok[i] == o["ok"] and it[i] == o["iterations"], i
+            assert np.array_equal(info[i], o["hard"][code.info_cols]), i
+    for dd in decs:
+        dd.close()
+
+
+def C_ptr(t):
+    import ctypes
+
+    return ctypes.c_void_p(t.data_ptr())
