@@ -13,7 +13,10 @@
  * the job completes.  A scheduler thread keeps up to `slots` launches in
  * flight: whenever a slot is free it packs every admitted, not yet launched
  * job (up to `max_launch_codewords`) into per-code groups and launches them, so
- * jobs that arrive while others run overlap with them on the device.
+ * jobs that arrive while others run overlap with them on the device.  Code
+ * groups whose BP batches share one flooding-kernel instance decode in one
+ * fused launch; the launch's decode units are spread over the streams of slots
+ * that are idle when it is issued.
  * While launches are in flight the scheduler thread polls their completion
  * events (yielding between polls: one host core), so completions are seen
  * within microseconds; an idle server sleeps on a condition variable.
@@ -33,8 +36,10 @@ extern "C" {
  * keeps them alive.  max_jobs_per_launch <= 0: unlimited.  Besides the pool,
  * each slot allocates its launch staging at creation, at its largest size:
  * per member max_launch_codewords x (4n + k + 2) bytes of device memory (+ 4n
- * for the OSD hand-off of BP-OSD members) and max_launch_codewords x
- * (ceil(k_max / 8) + 2) bytes of pinned results. */
+ * for the OSD hand-off of BP-OSD members; + 4 x edge slots x W for a member on
+ * the long-code kernel, W = its workgroups the GPU holds at once, times 2 with
+ * flooding damping, + 4n x W when n > 16 320) and max_launch_codewords x
+ * (ceil(k_max / 8) + 2) + 4 x members bytes of pinned results. */
 int sicc_server_create(const unsigned long long* decoders, int count, unsigned long long budget_bytes, int slots,
                        int max_jobs_per_launch, int max_launch_codewords, unsigned long long* out_handle);
 int sicc_server_destroy(unsigned long long handle);

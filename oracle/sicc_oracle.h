@@ -18,7 +18,9 @@
  *   oracle_build_ldpc       pinned: H*G^T = 0, rank, front-info layout.
  *   oracle_encode           pinned: syndrome of every output is zero, linearity.
  *   oracle_nms_decode       pinned: single-check closed form, tree code max-log
- *                           MAP by brute force, 1-iteration decode of clean codewords.
+ *                           MAP by brute force, 1-iteration decode of clean codewords,
+ *                           VN damping by the linear degree-2 closed form (two passes,
+ *                           damping-free fixed point).
  *   oracle_osd_decode       pinned: output in code, order-k OSD == brute-force ML,
  *                           argmin over explicit TEP set, basis independence, ties.
  *   oracle_ofdm_*           pinned: numpy FFT, closed-form QPSK LLR, brute force

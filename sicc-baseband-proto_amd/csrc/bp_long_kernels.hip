@@ -3,7 +3,9 @@
 // layered schedule.
 //
 // One workgroup of 16 wavefronts (8 for row degrees above 16) per codeword, persistent over
-// the batch.  APP values live in LDS when n * 4 bytes fit in 64 KiB (n <= 16 368),
+// the batch: as many workgroups as the GPU holds at once; after its first codeword
+// (its block index) each takes the next undecoded one from a global counter, so
+// the batch's tail balances over the workgroups that finish first.  APP values live in LDS when n * 4 bytes fit in 64 KiB (n <= 16 368),
 // otherwise in a per-workgroup slice of global scratch (L2 / MALL resident).
 // Check-to-variable messages r live in global scratch in a group-major,
 // position-minor layout: group g's rows are processed by consecutive threads,
@@ -47,13 +49,14 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
     extern __shared__ float lds_app[];
     __shared__ int s_flag[2];
     __shared__ float s_min[16];
+    __shared__ int s_next;
     const int tid = threadIdx.x;
     float* r = sc.r + static_cast<size_t>(blockIdx.x) * c.slots;
     float* qst = sc.q ? sc.q + static_cast<size_t>(blockIdx.x) * c.slots : nullptr;
     float* app = APP_LDS ? lds_app : sc.app + static_cast<size_t>(blockIdx.x) * c.n;
     const float alpha = p.alpha, offset = p.offset, damping = p.damping;
 
-    for (int cw = blockIdx.x; cw < io.num_cw; cw += gridDim.x) {
+    for (int cw = blockIdx.x; cw < io.num_cw;) {
         const float* llr = io.llr + static_cast<size_t>(cw) * c.n;
         for (int v = tid; v < c.n; v += kLongThreads) app[v] = llr[v];
         for (int e = tid; e < c.slots; e += kLongThreads) r[e] = 0.0f;
@@ -191,6 +194,9 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
             if (io.distance) io.distance[cw] = 0.0f;
         }
         __syncthreads();  // APP and flags are reused by the next codeword
+        if (tid == 0) s_next = static_cast<int>(gridDim.x) + atomicAdd(sc.work_ctr, 1);
+        __syncthreads();
+        cw = s_next;
     }
 }
 
@@ -219,16 +225,46 @@ void launch_width(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, con
 
 bool bp_long_app_fits_lds(int n) { return n > 0 && static_cast<size_t>(n) * sizeof(float) <= 65536 - 256; }
 
+namespace {
+template <int MAXW>
+int resident_width(const BpLongCodeDev& c, bool app_in_lds)
+{
+    const size_t lds = app_in_lds ? sizeof(float) * static_cast<size_t>(c.n) : 0;
+    constexpr int t = long_threads<MAXW>();
+    const void* k = c.layered ? (app_in_lds ? reinterpret_cast<const void*>(bp_long_kernel<MAXW, true, true>)
+                                            : reinterpret_cast<const void*>(bp_long_kernel<MAXW, true, false>))
+                              : (app_in_lds ? reinterpret_cast<const void*>(bp_long_kernel<MAXW, false, true>)
+                                            : reinterpret_cast<const void*>(bp_long_kernel<MAXW, false, false>));
+    return resident_blocks_per_cu(k, t, lds) * device_cu_count();
+}
+}  // namespace
+
+int bp_long_resident_blocks(const BpLongCodeDev& c, bool app_in_lds)
+{
+    int b = 0;
+    if (c.max_width <= 8)
+        b = resident_width<8>(c, app_in_lds);
+    else if (c.max_width <= 16)
+        b = resident_width<16>(c, app_in_lds);
+    else if (c.max_width <= 32)
+        b = resident_width<32>(c, app_in_lds);
+    else
+        throw std::invalid_argument("long BP: row degree above 32");
+    return b > 0 ? b : 1;
+}
+
 void launch_bp_long(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, const BpLongScratch& sc,
                     hipStream_t s)
 {
     if (io.num_cw <= 0) return;
-    if (!io.llr || !sc.r || sc.blocks <= 0) throw std::invalid_argument("long BP: missing input or scratch");
+    if (!io.llr || !sc.r || !sc.work_ctr || sc.blocks <= 0)
+        throw std::invalid_argument("long BP: missing input or scratch");
     if (!sc.app_in_lds && !sc.app) throw std::invalid_argument("long BP: missing APP scratch");
     if (sc.app_in_lds && !bp_long_app_fits_lds(c.n)) throw std::invalid_argument("long BP: APP does not fit LDS");
     if (!c.layered && p.damping != 0.0f && !sc.q) throw std::invalid_argument("long BP: damping needs q scratch");
     if (p.max_iter < 1) throw std::invalid_argument("long BP: max_iter must be positive");
     const int grid = io.num_cw < sc.blocks ? io.num_cw : sc.blocks;
+    launch_zero_i32(sc.work_ctr, 1, s);  // a kernel, not a memset node, under graph capture
     if (c.max_width <= 8)
         launch_width<8>(c, p, io, sc, grid, s);
     else if (c.max_width <= 16)

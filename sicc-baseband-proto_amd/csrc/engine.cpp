@@ -205,12 +205,14 @@ BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool
 {
     BpLongScratch ls;
     ls.app_in_lds = bp_long_app_fits_lds(code_.n);
-    // codewords in flight: four workgroups per CU when the APP sits in LDS (the
-    // tail of a batch balances better; N16200 decodes ~5 % faster than with two),
-    // two when it lives in global memory (more would only add scratch traffic)
-    ls.blocks = cus_ * (ls.app_in_lds ? 4 : 2);
+    // codewords in flight: the workgroups the GPU holds at once (each with its
+    // own message scratch slice); the kernel hands out the rest of the batch
+    // through a counter, so no workgroup waits for a slot to free up
+    ls.blocks = bp_long_resident_blocks(t.dev, ls.app_in_lds);
     const bool need_q = !layered && bp.damping != 0.0f;
+    sc.ensure(0, code_.n);  // the counters
     sc.ensure_long(ls.blocks, t.dev.slots, code_.n, need_q, !ls.app_in_lds);
+    ls.work_ctr = sc.counters.get() + 1;
     ls.r = sc.long_r.get();
     ls.q = need_q ? sc.long_q.get() : nullptr;
     ls.app = ls.app_in_lds ? nullptr : sc.long_app.get();

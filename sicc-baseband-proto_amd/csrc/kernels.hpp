@@ -178,13 +178,16 @@ struct BpLongCodeDev {
 // Per-workgroup scratch: r (and q for damped flooding) [blocks][slots]; APP
 // [blocks][n] unless it is kept in LDS.
 struct BpLongScratch {
-    float* r = nullptr;
-    float* q = nullptr;
-    float* app = nullptr;
-    int blocks = 0;
+    float* r = nullptr;       // [blocks][slots]
+    float* q = nullptr;       // [blocks][slots], flooding with damping
+    float* app = nullptr;     // [blocks][n] when the APP does not fit LDS
+    int* work_ctr = nullptr;  // one counter: codewords claimed past the first grid
+    int blocks = 0;           // workgroups (and scratch slices) of a launch
     bool app_in_lds = false;
 };
 bool bp_long_app_fits_lds(int n);
+// Workgroups of the long-code kernel for `c` the GPU holds at once.
+int bp_long_resident_blocks(const BpLongCodeDev& c, bool app_in_lds);
 void launch_bp_long(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, const BpLongScratch& sc,
                     hipStream_t s);
 

@@ -99,6 +99,37 @@ def test_demod_decode_matches_reference_decode_frame(golden, gated):
     assert np.array_equal(res["snr_gated"].astype(bool), gated & (rd[:, 2] < gate))
 
 
+def test_demod_decode_reproduces_ofdm_small_decode_frame():
+    """The single frame of tests/golden/ofdm_small.npz at its own size: the
+    reference decode_frame's information bits (df_info) and success (df_ok),
+    from the frame at start 0 with its CFO and noise variance (df_args)."""
+    import torch
+
+    print("[case] frame golden ofdm_small demod+decode", flush=True)
+    z = np.load(os.path.join(G, "ofdm_small.npz"))
+    phy = _fixture_phy(z)
+    cfo, nv, iters, alpha = (float(v) for v in z["df_args"])
+    frame = np.ascontiguousarray(z["df_frame"].astype(np.complex64))
+    dev = torch.device("cuda:0")
+    sync = np.zeros(1, sa.SYNC_DTYPE)
+    sync["metric"] = 1.0
+    sync["cfo_hz"] = cfo
+    sync["cfo_hz_d"] = cfo
+    d_rx = torch.from_numpy(frame.view(np.float32)).to(dev)
+    d_sync = torch.from_numpy(sync.view(np.uint8)).to(dev)
+    d_info = torch.empty((1, phy.blocks * 64), dtype=torch.uint8, device=dev)
+    d_res = torch.empty(sa.FRAME_RESULT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    with sa.OfdmReceiver(phy, 128, phy.blocks, noise_var=nv) as rxr, _ccsds_bp(int(iters), alpha) as dec, \
+            sa.FrameDecoder(rxr, dec) as fd:
+        fd.demod_decode_device(d_rx.data_ptr(), 1, frame.shape[0], d_sync.data_ptr(), d_info.data_ptr(),
+                               d_res.data_ptr())
+        torch.cuda.synchronize()
+    res = d_res.cpu().numpy().view(sa.FRAME_RESULT_DTYPE)
+    assert np.array_equal(d_info.cpu().numpy()[0], z["df_info"])
+    assert bool(res["parity_ok"][0]) == bool(z["df_ok"][0])
+    assert res["detected"][0] == 1 and res["blocks"][0] == phy.blocks
+
+
 def test_receive_decode_host_matches_reference(golden):
     """The whole chain from the buffers (this library's sync, then demod, BP and
     aggregation) through the host entry point: sync starts equal the
