@@ -403,6 +403,9 @@ def leg_served(a, sa, np, torch, dec128):
         decs.append(dec)
         pools.append(torch.from_numpy(pl).to(dev))
         ks.append(dec.k)
+        # each member's launch shape autotuned on its own pool, as for the mixed batch
+        torch.cuda.synchronize()
+        dec.autotune(pools[-1].data_ptr(), 1024)
     mj = a.served_mix_jobs
     mem = rng.integers(0, len(shapes), mj)
     cnt = rng.integers(1, 513, mj)
@@ -412,6 +415,7 @@ def leg_served(a, sa, np, torch, dec128):
     out["mix"] = {"job_codewords": "uniform 1..512", "codes": len(shapes),
                   "code": "QC-LDPC mix n 128..2048, rates 1/2, 11/16, 3/4, flooding BP 50 it",
                   "ebn0_db": {f"n{n}_mb{mb}": e for (n, mb), e in sorted(MIX_EBN0_SERVE.items())}}
+    out["mix"]["tunes"] = [d.get_tune() for d in decs]
     out["mix"].update(serve_sweep(a, sa, np, rng, decs, [int(x) for x in mem], cnt, mptr, ks))
     for d in decs:
         d.close()

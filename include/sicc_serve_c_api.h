@@ -37,8 +37,9 @@ extern "C" {
  * each slot allocates its launch staging at creation, at its largest size:
  * per member max_launch_codewords x (4n + k + 2) bytes of device memory (+ 4n
  * for the OSD hand-off of BP-OSD members; + 4 x edge slots x W for a member on
- * the long-code kernel, W = its workgroups the GPU holds at once, times 2 with
- * flooding damping, + 4n x W when n > 16 320) and max_launch_codewords x
+ * the long-code kernel, W = its workgroups per launch (1 024 for N16200, 256
+ * for N64800 on 256 CUs), times 2 with flooding damping, + 4n x W when
+ * n > 16 320) and max_launch_codewords x
  * (ceil(k_max / 8) + 2) + 4 x members bytes of pinned results. */
 int sicc_server_create(const unsigned long long* decoders, int count, unsigned long long budget_bytes, int slots,
                        int max_jobs_per_launch, int max_launch_codewords, unsigned long long* out_handle);

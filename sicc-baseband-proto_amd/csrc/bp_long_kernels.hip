@@ -3,9 +3,9 @@
 // layered schedule.
 //
 // One workgroup of 16 wavefronts (8 for row degrees above 16) per codeword, persistent over
-// the batch: as many workgroups as the GPU holds at once; after its first codeword
-// (its block index) each takes the next undecoded one from a global counter, so
-// the batch's tail balances over the workgroups that finish first.  APP values live in LDS when n * 4 bytes fit in 64 KiB (n <= 16 368),
+// the batch: after its first codeword (its block index) each workgroup takes the
+// next undecoded one from a global counter, so the batch's tail balances over
+// the workgroups that finish first, whatever the grid size.  APP values live in LDS when n * 4 bytes fit in 64 KiB (n <= 16 368),
 // otherwise in a per-workgroup slice of global scratch (L2 / MALL resident).
 // Check-to-variable messages r live in global scratch in a group-major,
 // position-minor layout: group g's rows are processed by consecutive threads,

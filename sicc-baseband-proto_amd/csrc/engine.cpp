@@ -205,10 +205,14 @@ BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool
 {
     BpLongScratch ls;
     ls.app_in_lds = bp_long_app_fits_lds(code_.n);
-    // codewords in flight: the workgroups the GPU holds at once (each with its
-    // own message scratch slice); the kernel hands out the rest of the batch
-    // through a counter, so no workgroup waits for a slot to free up
-    ls.blocks = bp_long_resident_blocks(t.dev, ls.app_in_lds);
+    // Workgroups of a launch, each with its own message scratch slice; after its
+    // first codeword each claims the next through a counter.  With the APP in
+    // LDS (about 64 KiB a workgroup) the occupancy query counts one workgroup per
+    // CU; a grid four times that lets the dispatcher start the others as LDS
+    // frees up (4096 N16200 frames: 9.8 ms at 1x, 9.3 at 2x, 8.5 at 4x, 9.4 at
+    // 8x).  With the APP in global memory more workgroups only add scratch
+    // traffic (1024 N64800 frames: 32.8 ms at 1x, 34.4 at 2x).
+    ls.blocks = bp_long_resident_blocks(t.dev, ls.app_in_lds) * (ls.app_in_lds ? 4 : 1);
     const bool need_q = !layered && bp.damping != 0.0f;
     sc.ensure(0, code_.n);  // the counters
     sc.ensure_long(ls.blocks, t.dev.slots, code_.n, need_q, !ls.app_in_lds);

@@ -16,14 +16,20 @@ struct Cf {
 static __device__ __forceinline__ Cd cmul(Cd a, Cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 static __device__ __forceinline__ Cf cmul(Cf a, Cf b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-// 1024-point (or any 4^k) forward FFT, Stockham radix-4, in LDS, in the
-// precision of C (Cd with double2 twiddles, Cf with float2 twiddles).
+// Forward FFT of any power of two N in [16, 2048], Stockham autosort in LDS,
+// in the precision of C (Cd with double2 twiddles, Cf with float2 twiddles),
+// with 256 threads.  Radix-4 stages while the span ns has four more to go, then
+// one radix-2 stage when log2 N is odd (N = 2 * 4^k: 32, 128, 512, 2048).
+// Stage of radix R over span ns: butterfly j < N / R takes buf[j + r N / R]
+// times w^(r k), k = j mod ns, w = exp(-2 pi i / (R ns)), and writes its R-point
+// DFT to (j / ns) R ns + k + r ns.  tw[m] = exp(-2 pi i m / N).
 // buf holds the input in natural order and receives the output in natural order.
 template <class C, class TW>
-static __device__ void fft_radix4(C* buf, const TW* tw, int N)
+static __device__ void fft_stockham(C* buf, const TW* tw, int N)
 {
     const int q = N >> 2;
-    for (int ns = 1; ns < N; ns <<= 2) {
+    int ns = 1;
+    for (; 4 * ns <= N; ns <<= 2) {
         // up to two butterflies per thread (N <= 2048 with 256 threads); fixed
         // indices keep the staged outputs in registers
         C v[2][4];
@@ -69,6 +75,21 @@ static __device__ void fft_radix4(C* buf, const TW* tw, int N)
         }
         __syncthreads();
     }
+    if (ns < N) {
+        // radix 2 over span ns = N / 2: butterfly j < ns reads and writes j and
+        // j + ns only (k = j, w^k = tw[j]), so it runs in place
+        for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+            const C a0 = buf[j];
+            const TW t = tw[j];
+            const C a1 = cmul(buf[j + ns], {t.x, t.y});
+            buf[j] = {a0.x + a1.x, a0.y + a1.y};
+            buf[j + ns] = {a0.x - a1.x, a0.y - a1.y};
+        }
+        __syncthreads();
+    }
 }
+
+// N is a power of two the LDS FFT takes (with 256 threads).
+static inline bool lds_fft_size_ok(int N) { return N >= 16 && N <= 2048 && (N & (N - 1)) == 0; }
 
 }  // namespace sicc

@@ -96,9 +96,8 @@ struct PipelineCore {
         if (cfg.bits_per_symbol != 1 && cfg.bits_per_symbol != 2 && cfg.bits_per_symbol != 4 &&
             cfg.bits_per_symbol != 6)
             throw ArgumentError("bits_per_symbol must be 1, 2, 4, or 6");
-        int q = cfg.nfft;
-        while (q > 1 && (q & 3) == 0) q >>= 2;
-        if (q != 1 || cfg.nfft < 16 || cfg.nfft > 1024) throw ArgumentError("nfft must be 4^k in [16, 1024]");
+        if (cfg.nfft < 16 || cfg.nfft > 2048 || (cfg.nfft & (cfg.nfft - 1)) != 0)
+            throw ArgumentError("nfft must be a power of two in [16, 2048]");
         if (cfg.cp < 0 || cfg.cp >= cfg.nfft) throw ArgumentError("cp must be in [0, nfft)");
         if (cfg.pre_half_len <= 0 || cfg.preamble_len != 2 * cfg.pre_half_len)
             throw ArgumentError("preamble_len must be 2 * pre_half_len");

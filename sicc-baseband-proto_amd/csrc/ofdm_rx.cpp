@@ -44,7 +44,8 @@ namespace {
 
 void check_common(const sicc_ofdm_config_t& cfg)
 {
-    if (cfg.nfft <= 0 || (cfg.nfft & (cfg.nfft - 1)) != 0) throw std::invalid_argument("nfft must be a power of two");
+    if (cfg.nfft < 16 || cfg.nfft > 2048 || (cfg.nfft & (cfg.nfft - 1)) != 0)
+        throw std::invalid_argument("nfft must be a power of two in [16, 2048]");
     if (cfg.cp < 0 || cfg.cp >= cfg.nfft) throw std::invalid_argument("cp must be in [0, nfft)");
     if (cfg.pre_half_len <= 0 || cfg.preamble_len != 2 * cfg.pre_half_len)
         throw std::invalid_argument("preamble_len must be 2 * pre_half_len (two identical halves)");

@@ -23,8 +23,8 @@
 // per candidate, window energies from the scan) / CFO.  Demod: one 256-thread
 // workgroup per (frame, pilot symbol) writes LS estimates (A floats2 per pilot,
 // L2-resident), then one per (frame, data symbol) derotates + strips the CP
-// while loading the FFT input, runs the radix-4 Stockham FFT in FP64 through
-// LDS (5 stages for 1024) and equalizes and turns the symbol into LLRs straight
+// while loading the FFT input, runs the Stockham FFT in FP64 through LDS
+// (radix 4, 5 stages for 1024; a last radix-2 stage for 512 / 2048) and equalizes and turns the symbol into LLRs straight
 // from LDS: the frequency grid never goes to HBM, each symbol window is read
 // once and each LLR written once, and a batch is tens of thousands of
 // independent workgroups.
@@ -390,7 +390,7 @@ __device__ void symbol_fft(const OfdmDemodArgs& a, const float2* x, double cfo, 
         c0 = cn;
     }
     __syncthreads();
-    fft_radix4(buf, a.twiddle, N);
+    fft_stockham(buf, a.twiddle, N);
 }
 
 // One workgroup per (frame, pilot symbol): LS estimate y conj(p) / max(|p|^2, 1e-9)
@@ -606,9 +606,7 @@ void launch_ofdm_sync(const OfdmSyncArgs& a, int frames, hipStream_t s)
 void launch_ofdm_demod(const OfdmDemodArgs& a, int frames, hipStream_t s)
 {
     if (frames <= 0) return;
-    int n = a.nfft;
-    while (n > 1 && (n & 3) == 0) n >>= 2;
-    if (n != 1 || a.nfft < 16 || a.nfft > 1024) throw std::invalid_argument("demod FFT needs nfft = 4^k in [16, 1024]");
+    if (!lds_fft_size_ok(a.nfft)) throw std::invalid_argument("demod FFT needs nfft a power of two in [16, 2048]");
     if (a.n_pilot < 1 && a.use_pilots) throw std::invalid_argument("pilot channel estimation needs a pilot symbol");
     if (a.use_pilots && !a.h_est) throw std::invalid_argument("demod: missing channel-estimate scratch");
     if (a.n_data < 1) throw std::invalid_argument("demod: no data symbol");

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void tx_encode_dvbs2_kernel(OfdmTxArgs a)
 __global__ __launch_bounds__(kThreads) void tx_symbol_kernel(OfdmTxArgs a)
 {
     // single-precision grid and inverse FFT (the frame is float IQ; the error of
-    // an FP32 radix-4 FFT, ~1e-7 of the frame rms, is far inside the frame tolerance)
+    // an FP32 Stockham FFT, ~1e-7 of the frame rms, is far inside the frame tolerance)
     extern __shared__ float2 shf[];
     Cf* buf = reinterpret_cast<Cf*>(shf);
     const int f = blockIdx.x / a.num_symbols;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void tx_symbol_kernel(OfdmTxArgs a)
         buf[a.used_natural[sc]] = {v.x, -v.y};
     }
     __syncthreads();
-    fft_radix4(buf, a.twiddle, N);
+    fft_stockham(buf, a.twiddle, N);
     const float scale = a.amplitude / static_cast<float>(N);
     float2* o = out + a.preamble_len + static_cast<size_t>(sym) * (N + a.cp);
     for (int i = threadIdx.x; i < N + a.cp; i += blockDim.x) {
@@ -222,9 +222,7 @@ __global__ __launch_bounds__(kThreads) void tx_symbol_kernel(OfdmTxArgs a)
 void launch_ofdm_tx(const OfdmTxArgs& a, int frames, hipStream_t s)
 {
     if (frames <= 0) return;
-    int n = a.nfft;
-    while (n > 1 && (n & 3) == 0) n >>= 2;
-    if (n != 1 || a.nfft < 16 || a.nfft > 1024) throw std::invalid_argument("TX FFT needs nfft = 4^k in [16, 1024]");
+    if (!lds_fft_size_ok(a.nfft)) throw std::invalid_argument("TX FFT needs nfft a power of two in [16, 2048]");
     if (a.cp < 0 || a.cp > a.nfft) throw std::invalid_argument("TX cp out of range");
     if (a.code_k <= 0 || a.code_n <= a.code_k || a.blocks <= 0 ||
         static_cast<long long>(a.blocks) * a.code_n > a.coded_bits)

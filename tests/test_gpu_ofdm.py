@@ -1,7 +1,7 @@
 """GPU parity of the HIP OFDM receiver (sync + fused demod) against the CPU oracle.
 
 Sync: start index identical, CFO within 1e-6 relative.  LLRs: the FFT is FP64
-on both sides but by different algorithms (radix-4 Stockham vs the reference's
+on both sides but by different algorithms (radix-4/2 Stockham vs the reference's
 radix-2 with a twiddle recurrence), so values rounded to float can differ by
 an ulp; the stated bound is |gpu - cpu| <= 1e-5 * |cpu| + 1e-5 * rms(cpu),
 and hard decisions must agree on >= 99.99 % of bits.
@@ -18,8 +18,9 @@ import waveform as wf
 pytestmark = pytest.mark.gpu
 
 
-def _ofdm_batch(frames, seed, bits=4, snr_db=14.0, fading=True, num_symbols=14, active_sc=600):
-    phy = wf.OfdmPhy(bits_per_symbol=bits, active_sc=active_sc, num_symbols=num_symbols)
+def _ofdm_batch(frames, seed, bits=4, snr_db=14.0, fading=True, num_symbols=14, active_sc=600, nfft=1024):
+    phy = wf.OfdmPhy(bits_per_symbol=bits, active_sc=active_sc, num_symbols=num_symbols, nfft=nfft,
+                     cp=72 * nfft // 1024 if nfft >= 256 else 18, pre_half_len=nfft // 2)
     rng = np.random.default_rng(seed)
     coded = rng.integers(0, 2, (frames, phy.coded_bits), dtype=np.uint8)
     tx = wf.tx_frames(phy, coded)
@@ -63,6 +64,41 @@ def test_receive_matches_oracle(bits):
             _check_llr(llr[f], ref[: blocks * n])
         if thr == 0.2:
             assert (sync["start"] >= 0).all()
+
+
+@pytest.mark.parametrize("nfft,active_sc", [(64, 36), (128, 72), (512, 300), (2048, 1200)])
+def test_receive_fft_sizes(nfft, active_sc):
+    """FFT sizes other than 1024: 4^k (64) and 2 * 4^k (128, 512, 2048, which end
+    with a radix-2 stage); sync and LLRs against the oracle (radix-2 reference
+    FFT) at the bound of the 1024-point tests."""
+    print(f"[case] OFDM receive nfft={nfft}", flush=True)
+    phy, coded, ch = _ofdm_batch(6, 700 + nfft, bits=4, snr_db=16.0, nfft=nfft, active_sc=active_sc)
+    n = 256
+    blocks = phy.coded_bits // n
+    search = 256
+    with sa.OfdmReceiver(phy, n, blocks, max_search=search, noise_var=ch.noise_var_freq, threshold=0.2) as rxr:
+        sync, llr = rxr.receive_host(ch.rx)
+    found = 0
+    for f in range(ch.rx.shape[0]):
+        ok, start, peak, cfo, _ = oracle.find_frame(ch.rx[f], phy.pre_half_len, phy.preamble, min(16, phy.cp), 0.2,
+                                                    phy.fs, max_search=search)
+        assert sync["start"][f] == start, (f, sync["start"][f], start)
+        if not ok:
+            continue
+        found += 1
+        assert sync["cfo_hz_d"][f] == pytest.approx(cfo, rel=1e-6)
+        ref = oracle.demod_frame(ch.rx[f][start:start + phy.frame_len], phy, sync["cfo_hz_d"][f], ch.noise_var_freq)
+        _check_llr(llr[f], ref[: blocks * n])
+    assert found >= 4
+
+
+def test_fft_size_limits():
+    phy = wf.OfdmPhy(nfft=4096, cp=288, pre_half_len=2048, active_sc=2400)
+    with pytest.raises(sa.SiccError):
+        sa.OfdmReceiver(phy, 1024, 8)
+    phy = wf.OfdmPhy(nfft=8, cp=2, pre_half_len=4, active_sc=4, num_symbols=3)
+    with pytest.raises(sa.SiccError):
+        sa.OfdmReceiver(phy, 16, 1)
 
 
 @pytest.mark.parametrize("max_search", [256, 3000, 0])

@@ -25,9 +25,10 @@ pytestmark = pytest.mark.gpu
 N, K = 128, 64
 
 
-def _phy(bits=2, active_sc=512, num_symbols=6, pilots=(0, 3)):
+def _phy(bits=2, active_sc=512, num_symbols=6, pilots=(0, 3), nfft=1024):
     return wf.OfdmPhy(bits_per_symbol=bits, active_sc=active_sc, num_symbols=num_symbols,
-                      pilot_symbols=np.array(pilots, np.int32))
+                      pilot_symbols=np.array(pilots, np.int32), nfft=nfft, cp=max(72 * nfft // 1024, 8),
+                      pre_half_len=nfft // 2)
 
 
 def _tables(phy, gen=None, dvb=None, precomp=None, actual=None):
@@ -89,6 +90,21 @@ def test_tx_matches_oracle_generator_channel_and_noise():
         p.synchronize_tx(0)
         _close(p.read_frame(d), ref)
         p.close()
+
+
+@pytest.mark.parametrize("nfft,active_sc", [(128, 64), (512, 256), (2048, 1024)])
+def test_tx_fft_sizes_match_oracle(nfft, active_sc):
+    """The transmitter's inverse FFT at sizes other than 1024 (ending with a
+    radix-2 stage), against the oracle TX at the frame tolerance."""
+    print(f"[case] pipeline TX nfft={nfft}", flush=True)
+    phy = _phy(active_sc=active_sc, nfft=nfft)
+    gen = _code().generator()
+    p, blocks = _pipeline(phy, amp=0.25, gen=gen)
+    rng = np.random.default_rng(nfft)
+    info = rng.integers(0, 2, blocks * K, dtype=np.uint8)
+    _, ref = oracle.tx_frame(info, N, K, phy, 0.25, gen_kn=gen)
+    _close(p.submit_tx_frame(1, info), ref)
+    p.close()
 
 
 def test_tx_dvbs2_encoder_matches_oracle():
