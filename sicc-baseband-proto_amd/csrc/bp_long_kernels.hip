@@ -263,7 +263,12 @@ void launch_bp_long(const BpLongCodeDev& c, const BpParams& p, const BpIo& io, c
     if (sc.app_in_lds && !bp_long_app_fits_lds(c.n)) throw std::invalid_argument("long BP: APP does not fit LDS");
     if (!c.layered && p.damping != 0.0f && !sc.q) throw std::invalid_argument("long BP: damping needs q scratch");
     if (p.max_iter < 1) throw std::invalid_argument("long BP: max_iter must be positive");
-    const int grid = io.num_cw < sc.blocks ? io.num_cw : sc.blocks;
+    // about four codewords a workgroup, between the resident count and the
+    // scratch slices (4096 N16200 frames: 1024 workgroups, 1024 frames: 256)
+    int grid = (io.num_cw + 3) / 4;
+    if (grid < sc.resident) grid = sc.resident;
+    if (grid > sc.blocks) grid = sc.blocks;
+    if (grid > io.num_cw) grid = io.num_cw;
     launch_zero_i32(sc.work_ctr, 1, s);  // a kernel, not a memset node, under graph capture
     if (c.max_width <= 8)
         launch_width<8>(c, p, io, sc, grid, s);

@@ -208,11 +208,14 @@ BpLongScratch Engine::long_scratch(const LongTables& t, const BpParams& bp, bool
     // Workgroups of a launch, each with its own message scratch slice; after its
     // first codeword each claims the next through a counter.  With the APP in
     // LDS (about 64 KiB a workgroup) the occupancy query counts one workgroup per
-    // CU; a grid four times that lets the dispatcher start the others as LDS
-    // frees up (4096 N16200 frames: 9.8 ms at 1x, 9.3 at 2x, 8.5 at 4x, 9.4 at
-    // 8x).  With the APP in global memory more workgroups only add scratch
-    // traffic (1024 N64800 frames: 32.8 ms at 1x, 34.4 at 2x).
-    ls.blocks = bp_long_resident_blocks(t.dev, ls.app_in_lds) * (ls.app_in_lds ? 4 : 1);
+    // CU; up to four times that many let the dispatcher start the others as LDS
+    // frees up, and about four codewords a workgroup balance best (4096 N16200
+    // frames: 9.8 ms on 1x, 9.3 on 2x, 8.5 on 4x, 9.4 on 8x the count; 1024
+    // frames: 4.24 ms on 1x, 4.48 on 4x).  With the APP in global memory more
+    // workgroups only add scratch traffic (1024 N64800 frames: 32.8 ms on 1x,
+    // 34.4 on 2x).
+    ls.resident = bp_long_resident_blocks(t.dev, ls.app_in_lds);
+    ls.blocks = ls.resident * (ls.app_in_lds ? 4 : 1);
     const bool need_q = !layered && bp.damping != 0.0f;
     sc.ensure(0, code_.n);  // the counters
     sc.ensure_long(ls.blocks, t.dev.slots, code_.n, need_q, !ls.app_in_lds);

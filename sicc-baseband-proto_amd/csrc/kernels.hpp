@@ -182,7 +182,8 @@ struct BpLongScratch {
     float* q = nullptr;       // [blocks][slots], flooding with damping
     float* app = nullptr;     // [blocks][n] when the APP does not fit LDS
     int* work_ctr = nullptr;  // one counter: codewords claimed past the first grid
-    int blocks = 0;           // workgroups (and scratch slices) of a launch
+    int blocks = 0;           // scratch slices: the most workgroups of a launch
+    int resident = 0;         // workgroups the occupancy query counts the GPU holding
     bool app_in_lds = false;
 };
 bool bp_long_app_fits_lds(int n);
