@@ -41,11 +41,16 @@ HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # CUs x bytes/clk x GHz
 VALU_PEAK_GIPS = 256 * 2.4  # wave64 VALU instructions issued per s (one per CU per clock), in G
 
-# HBM bytes and VALU instructions per headline bp_flood_kernel dispatch, from the
-# rocprofv3 counter passes summarized in profiles/bp_flood_ccsds128_1M_pmc.md
-# (FETCH_SIZE x2 and WRITE_SIZE x1 after calibration on a copy of known size);
-# they hold for the profiled workload only.
+# Counter figures per headline bp_flood_kernel dispatch, from the rocprofv3
+# passes summarized in profiles/bp_flood_ccsds128_1M_pmc.md: HBM bytes
+# (FETCH_SIZE x2 and WRITE_SIZE x1 after calibration on a copy of known size),
+# VALU instructions, LDS bytes (SQ_INSTS_LDS_LOAD/STORE_BANDWIDTH x 64 B), LDS
+# busy and bank-conflict cycles and mean waves per CU.  A profiler run cannot
+# sit inside the timed region, so the timed launch divides these counts; they
+# hold for the profiled workload only.
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "bp_flood_ccsds128_1M_pmc.md")
+PMC_KEYS = ("traffic_bytes_per_launch", "valu_insts_per_launch", "lds_bytes_per_launch",
+            "lds_bank_conflict_cycles_per_launch", "lds_active_cycles_per_launch", "mean_waves_per_cu")
 
 
 def profiled_counts(a, B):
@@ -54,10 +59,10 @@ def profiled_counts(a, B):
     got = {}
     with open(PMC_SUMMARY) as f:
         for line in f:
-            for key in ("traffic_bytes_per_launch", "valu_insts_per_launch"):
+            for key in PMC_KEYS:
                 if line.startswith(key + ":"):
-                    got[key] = int(line.split(":")[1])
-    return got if len(got) == 2 else None
+                    got[key] = float(line.split(":")[1])
+    return got if len(got) == len(PMC_KEYS) else None
 
 # Eb/N0 (dB) per (n, block rows) of the mixed-code workload where the frame error
 # rate of 50-iteration flooding min-sum (alpha 0.75) crosses 1e-2 (mixed leg) and
@@ -160,12 +165,11 @@ def leg_headline(a, sa, np, torch, stream):
     bp_ms = kt["bp_ms"] / max(kt["bp_launches"], 1)
     osd_ms = kt["osd_ms"] / max(kt["osd_launches"], 1)
     hbm_bytes = B * (4 * dec.n + dec.k + 1)
-    lds_bytes = 16 * dec.nnz * float(iters.sum())
     res = {
         "value": value, "ms_per_step": ms_per_step, "B": B, "k": dec.k,
         "fer": frame_err / B, "ber": int((got != info).sum()) / (B * dec.k), "frame_errors": frame_err,
         "osd_frames": int(d_used.sum().item()), "mean_bp_iterations": float(iters.mean()),
-        "bp_ms": bp_ms, "osd_ms": osd_ms, "hbm_bytes": hbm_bytes, "lds_bytes": lds_bytes, "tune": tune,
+        "bp_ms": bp_ms, "osd_ms": osd_ms, "hbm_bytes": hbm_bytes, "tune": tune,
     }
     # plain: one 256-codeword launch per call, host sync per call
     PB = 256
@@ -912,14 +916,22 @@ def main() -> int:
         cpu, cpu_ref = cpu_baselines(a, sa, np, llr128, got128)
 
     bp_gbs = h["hbm_bytes"] / (h["bp_ms"] * 1e-3) / 1e9
-    lds_gbs = h["lds_bytes"] / (h["bp_ms"] * 1e-3) / 1e9
     pmc = profiled_counts(a, h["B"])
-    valu = None
+    valu = lds = occupancy = None
     if pmc:
+        src = "profiles/bp_flood_ccsds128_1M_pmc.md (counts per launch), divided by this run's launch time"
         gips = pmc["valu_insts_per_launch"] / (h["bp_ms"] * 1e-3) / 1e9
         valu = {"achieved": round(gips, 1), "peak": VALU_PEAK_GIPS, "unit": "G wave-instructions/s",
-                "frac": round(gips / VALU_PEAK_GIPS, 4), "insts_per_launch": pmc["valu_insts_per_launch"],
-                "source": "profiles/bp_flood_ccsds128_1M_pmc.md"}
+                "frac": round(gips / VALU_PEAK_GIPS, 4), "insts_per_launch": int(pmc["valu_insts_per_launch"]),
+                "source": src}
+        lds_gbs = pmc["lds_bytes_per_launch"] / (h["bp_ms"] * 1e-3) / 1e9
+        busy = pmc["lds_active_cycles_per_launch"]
+        lds = {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
+               "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "bytes_per_launch": int(pmc["lds_bytes_per_launch"]),
+               "bank_conflict_frac_of_busy": round(pmc["lds_bank_conflict_cycles_per_launch"] / busy, 4),
+               "source": src}
+        occupancy = {"mean_waves_per_cu": pmc["mean_waves_per_cu"], "max_waves_per_cu": 32,
+                     "frac": round(pmc["mean_waves_per_cu"] / 32, 4), "source": src}
     out = {
         "metric": "decoded info Mbps at FER<=1e-3 (batched vs plain); p50/p99 latency",
         "value": round(h["value"], 3),
@@ -943,11 +955,11 @@ def main() -> int:
         "batched_vs_plain": round(h["value"] / h["plain"]["value"], 2) if h.get("plain") else None,
         "roofline": {"bound": "hbm", "achieved": round(bp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bp_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": pmc["traffic_bytes_per_launch"] if pmc else None, "kernel": "bp_flood_kernel_lean",
+                     "traffic": int(pmc["traffic_bytes_per_launch"]) if pmc else None, "kernel": "bp_flood_kernel_lean",
                      "kernel_ms": round(h["bp_ms"], 4), "bytes_per_launch": h["hbm_bytes"]},
-        "lds_roofline": {"achieved": round(lds_gbs, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
-                         "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "bytes_per_launch": h["lds_bytes"]},
+        "lds_roofline": lds,
         "valu_roofline": valu,
+        "occupancy": occupancy,
         "osd_kernel_ms": round(h["osd_ms"], 4),
         "cpu_baseline": cpu,
         "cpu_reference_min_sum": cpu_ref,
