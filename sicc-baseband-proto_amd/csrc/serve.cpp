@@ -13,6 +13,7 @@
 #include "../../include/sicc_serve_c_api.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <condition_variable>
 #include <cstring>
@@ -139,14 +140,23 @@ public:
 
     ~Server()
     {
+        stop_thread();
+        release();
+    }
+
+    // Stops and joins the scheduler thread (idempotent).  Also run at process
+    // exit for servers still open, so that no scheduler thread is inside a HIP
+    // call while the runtime tears down.
+    void stop_thread()
+    {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
         }
         wake_.store(true, std::memory_order_release);
         cv_.notify_all();
+        std::lock_guard<std::mutex> jl(join_mu_);
         if (thread_.joinable()) thread_.join();
-        release();
     }
 
 private:
@@ -784,6 +794,7 @@ private:
     double max_wait_us_ = 0.0;
     bool failed_ = false;
     std::string fail_msg_;
+    std::mutex join_mu_;
     std::thread thread_;
 };
 
@@ -793,6 +804,17 @@ HandleTable<Server>& stable()
     // runtime from static destructors
     static HandleTable<Server>& t = *new HandleTable<Server>;
     return t;
+}
+
+// Registered after the first server exists, hence after the HIP runtime set up
+// its own exit handlers: runs before them and stops every open server's
+// scheduler thread (their device resources are left to the process exit).
+void stop_open_servers() { stable().for_each([](Server& s) { s.stop_thread(); }); }
+
+void register_exit_stop()
+{
+    static std::once_flag once;
+    std::call_once(once, [] { std::atexit(stop_open_servers); });
 }
 
 std::shared_ptr<Server> slookup(unsigned long long h)
@@ -826,6 +848,7 @@ int sicc_server_create(const unsigned long long* decoders, int count, unsigned l
         }
         *out_handle = stable().add(std::make_unique<Server>(std::move(members), static_cast<size_t>(budget_bytes), slots,
                                                             max_jobs_per_launch, max_launch_codewords));
+        register_exit_stop();
         return CUDA_OSD_STATUS_OK;
     });
 }

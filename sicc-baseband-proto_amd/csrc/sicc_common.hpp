@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <vector>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -54,6 +55,17 @@ public:
         std::lock_guard<std::mutex> lk(mu_);
         auto it = map_.find(h);
         return it == map_.end() ? nullptr : it->second;
+    }
+    // Calls f on every live object, outside the table's lock.
+    template <class F>
+    void for_each(F f)
+    {
+        std::vector<std::shared_ptr<T>> all;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto& kv : map_) all.push_back(kv.second);
+        }
+        for (auto& o : all) f(*o);
     }
     bool remove(uint64_t h)
     {

@@ -163,6 +163,34 @@ def test_ragged_info_width_round_trip():
     dec.close()
 
 
+def test_exit_with_servers_open():
+    """A process that exits with a server still open (jobs queued, launches in
+    flight, handle never destroyed) exits cleanly: the library stops every open
+    server's scheduler thread before the HIP runtime tears down."""
+    import os
+    import subprocess
+    import sys
+
+    print("[case] server: process exit with open servers", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path[:0] = [%r, %r]\n"
+        "import numpy as np, oracle, sicc_amd as sa\n"
+        "n, m, rp, ci = oracle.alist_parse(sa.CCSDS_128_64)\n"
+        "dec = sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0)\n"
+        "srv = sa.Server([dec], budget_bytes=64 << 20, slots=2, max_launch_codewords=4096)\n"
+        "srv2 = sa.Server([dec], budget_bytes=64 << 20, slots=1, max_launch_codewords=4096)\n"
+        "rng = np.random.default_rng(5)\n"
+        "for j in range(200):\n"
+        "    srv.submit(j, 0, rng.standard_normal((64, 128)).astype(np.float32) + 2.0)\n"
+        "srv2.submit(0, 0, np.ones((8, 128), np.float32))\n"
+        "print('submitted', flush=True)\n"
+    ) % (os.path.join(root, "sicc-baseband-proto_amd"), os.path.join(root, "oracle"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    assert "submitted" in r.stdout
+
+
 def test_failed_create_is_clean():
     """A budget the device cannot hold fails server creation with an error status
     (everything allocated before the failure is released), the failure does not
