@@ -226,9 +226,12 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
 
     // Codeword assignment: static striding, or chunks taken from a global
     // counter (work stealing) so teams that finish early take more work.
+    // A single-wave team claims its next chunk when the current chunk's last
+    // codeword starts (nb), so that chunk's first LLRs load while it decodes,
+    // as under static striding.
     const bool steal = io.work_ctr != nullptr;
     const int chunk = io.chunk > 0 ? io.chunk : 1;
-    int cw = gteam, lim = 0;
+    int cw = gteam, lim = 0, nb = 0;
     if (steal) {
         int b = 0;
         if (TEAM == 1) {
@@ -265,7 +268,16 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
 #pragma unroll
         for (int t = 0; t < NPL; ++t) lr[t] = lrn[t];
         {
-            const int nx = steal ? (cw + 1 < lim ? cw + 1 : io.num_cw) : cw + nteams;
+            int nx = cw + nteams;
+            if (steal) {
+                nx = cw + 1 < lim ? cw + 1 : io.num_cw;
+                if (TEAM == 1 && cw + 1 >= lim) {
+                    int b = 0;
+                    if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
+                    nb = __builtin_amdgcn_readlane(b, 0);
+                    nx = nb;
+                }
+            }
             if (nx < io.num_cw) {
                 const float* Ln = io.llr + static_cast<size_t>(nx) * n;
 #pragma unroll
@@ -507,12 +519,13 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
         team_sync<TEAM>();
         if (!steal) {
             cw += nteams;
-        } else if (++cw >= lim) {
+        } else if (TEAM == 1 && ++cw >= lim) {
+            // the chunk claimed ahead; its first LLRs are already in lrn
+            cw = nb;
+            lim = min(nb + chunk, io.num_cw);
+        } else if (TEAM > 1 && ++cw >= lim) {
             int b = 0;
-            if (TEAM == 1) {
-                if (lane == 0) b = atomicAdd(io.work_ctr, chunk);
-                b = __builtin_amdgcn_readlane(b, 0);
-            } else {
+            {
                 if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
                 __syncthreads();
                 b = red[4];
@@ -578,14 +591,15 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_multi_ke
 // damping (the (128,64) class): its vector registers fit eight waves per SIMD,
 // but the kernel arguments held in scalar registers cap residency at seven.
 // Two ways to the eighth wave:
-//  * bp_flood_kernel_lean takes only the launch's hard-decision outputs and no
-//    work-stealing cursor; with the soft outputs' code gone its arguments fit
-//    the scalar budget outright;
+//  * bp_flood_kernel_lean takes only the launch's hard-decision outputs (and
+//    the work-stealing cursor); with the soft outputs' code gone its arguments
+//    fit the scalar budget outright;
 //  * bp_flood_kernel_w8 (any outputs) caps the scalar budget, spilling some
 //    arguments to vector lanes.
 struct BpIoLean {
     const float* llr;
     int num_cw;
+    int chunk;
     uint8_t* info_bits;
     uint8_t* success;
     int* iterations;
@@ -593,9 +607,15 @@ struct BpIoLean {
     float* osd_llr;
     int* osd_index;
     int* osd_count;
+    int* work_ctr;
 };
 
-__global__ __launch_bounds__(256) void bp_flood_kernel_lean(BpCodeDev code, BpParams p, BpIoLean l, int e_pad)
+// STEAL: the build with the work-stealing cursor needs a few more registers;
+// it is held to eight waves per SIMD (a 12-byte spill), the static build is not
+// constrained (62 registers, no spill).
+template <bool STEAL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEAL ? 8 : 1))) void bp_flood_kernel_lean(
+    BpCodeDev code, BpParams p, BpIoLean l, int e_pad)
 {
     BpIo io;
     io.llr = l.llr;
@@ -607,6 +627,8 @@ __global__ __launch_bounds__(256) void bp_flood_kernel_lean(BpCodeDev code, BpPa
     io.osd_llr = l.osd_llr;
     io.osd_index = l.osd_index;
     io.osd_count = l.osd_count;
+    io.work_ctr = STEAL ? l.work_ctr : nullptr;
+    io.chunk = l.chunk;
     bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad, grid_team<64>(), grid_teams<64>());
 }
 
@@ -635,11 +657,13 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     bool lean = false;
     if constexpr (MPL == 1 && NPL == 2 && MAXDC == 8 && TEAM == 1) {
         if (!damp) {
-            lean = !io.code_bits && !io.app && !io.distance && !io.work_ctr;
+            lean = !io.code_bits && !io.app && !io.distance;
             kern = bp_flood_kernel_w8;
         }
     }
-    const void* kfn = lean ? reinterpret_cast<const void*>(bp_flood_kernel_lean) : reinterpret_cast<const void*>(kern);
+    const void* lean_fn = io.work_ctr ? reinterpret_cast<const void*>(bp_flood_kernel_lean<true>)
+                                      : reinterpret_cast<const void*>(bp_flood_kernel_lean<false>);
+    const void* kfn = lean ? lean_fn : reinterpret_cast<const void*>(kern);
     // A persistent grid larger than what the GPU holds at once would leave its
     // last teams to start after the first ones finished all their work.
     const long resident_teams = static_cast<long>(resident_blocks_per_cu(kfn, T * tpb, lds)) * device_cu_count() * tpb;
@@ -648,9 +672,12 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     if (teams < 1) teams = 1;
     const int blocks = static_cast<int>((teams + tpb - 1) / tpb);
     if (lean) {
-        const BpIoLean l{io.llr, io.num_cw, io.info_bits, io.success, io.iterations,
-                         io.used_osd, io.osd_llr, io.osd_index, io.osd_count};
-        hipLaunchKernelGGL(bp_flood_kernel_lean, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
+        const BpIoLean l{io.llr,     io.num_cw,   io.chunk,     io.info_bits, io.success, io.iterations,
+                         io.used_osd, io.osd_llr, io.osd_index, io.osd_count, io.work_ctr};
+        if (io.work_ctr)
+            hipLaunchKernelGGL(bp_flood_kernel_lean<true>, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
+        else
+            hipLaunchKernelGGL(bp_flood_kernel_lean<false>, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
     } else {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, io, e_pad);
     }
