@@ -82,6 +82,51 @@ def test_alone_equals_in_mix():
         d.close()
 
 
+def test_full_spec_mix_equals_each_decoder_alone():
+    """The served workload's shape: the 15-code QC mix (n 128..2048, three
+    rates each), jobs of 1..512 codewords, a 16 GB budget, four slots and
+    65 536-codeword launches (fused code groups, launches spread over idle
+    slots).  250 jobs released as one burst: every job's bits, success and
+    OSD flags equal the member decoder's own decode of that job, and a sample
+    equals the oracle."""
+    print("[case] server: full-spec 15-code mix, 250 jobs of 1..512 codewords", flush=True)
+    rng = np.random.default_rng(515)
+    shapes = [(n, mb) for n in (128, 256, 512, 1024, 2048) for mb in (8, 5, 4)]
+    decs, codes = [], []
+    for n, mb in shapes:
+        m, rp, ci = sa.qc_code(n, mb, 3, seed=1000 + n + mb)
+        decs.append(sa.Decoder.bp_csr(n, m, rp, ci, max_iter=50, alpha=0.75, schedule=0))
+        codes.append(oracle.Code(n, m, rp, ci, True))
+    jobs = []
+    for j in range(250):
+        mem = int(rng.integers(0, len(shapes)))
+        cnt = int(rng.integers(1, 513))
+        c = codes[mem]
+        info = rng.integers(0, 2, (cnt, c.k), dtype=np.uint8)
+        jobs.append((mem, oracle.awgn_llr(c.encode(info), 2.6, c.k / c.n, rng)))
+    with sa.Server(decs, budget_bytes=16 << 30, slots=4, max_launch_codewords=1 << 16) as srv:
+        srv.hold(True)
+        for j, (mem, llr) in enumerate(jobs):
+            srv.submit(j, mem, llr)
+        srv.hold(False)
+        srv.drain()
+        st = srv.stats()
+        assert st.jobs_completed == len(jobs) and st.launches < len(jobs)
+        for j, (mem, llr) in enumerate(jobs):
+            info, ok, used, _ = srv.fetch(j, llr.shape[0], codes[mem].k)
+            ref = decs[mem].decode_host(llr)
+            assert np.array_equal(info, ref["info"]), j
+            assert np.array_equal(ok.astype(bool), ref["bp_ok"].astype(bool)), j
+            assert not used.any(), j
+            if j < 12:
+                for i in range(0, llr.shape[0], 37):
+                    o = codes[mem].nms(llr[i], max_iter=50, alpha=0.75)
+                    assert np.array_equal(info[i], o["hard"][codes[mem].info_cols]), (j, i)
+                    assert bool(ok[i]) == o["ok"], (j, i)
+    for d in decs:
+        d.close()
+
+
 def test_budget_admission_and_plain_mode():
     decs, codes = _members()
     jobs = _jobs(codes, 30, 4)
