@@ -390,7 +390,7 @@ __device__ void symbol_fft(const OfdmDemodArgs& a, const float2* x, double cfo, 
         c0 = cn;
     }
     __syncthreads();
-    fft_stockham(buf, a.twiddle, N);
+    fft_stockham<false>(buf, a.twiddle, N);
 }
 
 // One workgroup per (frame, pilot symbol): LS estimate y conj(p) / max(|p|^2, 1e-9)

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void tx_symbol_kernel(OfdmTxArgs a)
         buf[a.used_natural[sc]] = {v.x, -v.y};
     }
     __syncthreads();
-    fft_stockham(buf, a.twiddle, N);
+    fft_stockham<true>(buf, a.twiddle, N);
     const float scale = a.amplitude / static_cast<float>(N);
     float2* o = out + a.preamble_len + static_cast<size_t>(sym) * (N + a.cp);
     for (int i = threadIdx.x; i < N + a.cp; i += blockDim.x) {
