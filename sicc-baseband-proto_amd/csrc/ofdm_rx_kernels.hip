@@ -470,6 +470,7 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
     const float2* h0p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p0) * A : nullptr;
     const float2* h1p = a.use_pilots ? a.h_est + (static_cast<size_t>(f) * a.n_pilot + p1) * A : nullptr;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};  // |h|^2, |y - h hard|^2, |eq0 - hard|^2, |hard|^2
+    const bool vec_ok = (reinterpret_cast<uintptr_t>(out) & (B == 4 ? 15u : 7u)) == 0;
     for (int sc = threadIdx.x; sc < A; sc += blockDim.x) {
         const int obit = (di * A + sc) * B;
         if (!st && obit >= a.llr_per_frame) continue;  // pad resource elements
@@ -540,6 +541,7 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
         for (int g = 0; g < LI; ++g) mx = dxs[g] < mx ? dxs[g] : mx;
 #pragma unroll
         for (int g = 0; g < LQ; ++g) my = dys[g] < my ? dys[g] : my;
+        float lv[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             float d0, d1;
@@ -566,7 +568,25 @@ __global__ __launch_bounds__(kThreads) void ofdm_data_kernel(OfdmDemodArgs a)
                 d0 = mx + m0;
                 d1 = mx + m1;
             }
-            if (obit + b < a.llr_per_frame) out[obit + b] = static_cast<float>(static_cast<double>(d1 - d0) * inv_s2);
+            lv[b] = static_cast<float>(static_cast<double>(d1 - d0) * inv_s2);
+        }
+        // a resource element's B LLRs leave as one vector store (B floats at a
+        // multiple of B: 16-byte aligned for B = 4, 8-byte for B = 2 and 6, when
+        // the frame's LLR row starts so aligned)
+        if (obit + B <= a.llr_per_frame && vec_ok) {
+            if (B == 4) {
+                *reinterpret_cast<float4*>(out + obit) = make_float4(lv[0], lv[1 % B], lv[2 % B], lv[3 % B]);
+            } else if (B == 2 || B == 6) {
+#pragma unroll
+                for (int b = 0; b < B; b += 2)
+                    *reinterpret_cast<float2*>(out + obit + b) = make_float2(lv[b], lv[(b + 1) % B]);
+            } else {
+                out[obit] = lv[0];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if (obit + b < a.llr_per_frame) out[obit + b] = lv[b];
         }
     }
     if (st) {
