@@ -112,7 +112,7 @@ struct Slot {
     hipEvent_t gathered = nullptr;        // the launch's packed LLRs are in place
     std::vector<hipEvent_t> lane_done;    // per lane stream: the launch's last group there
     bool busy = false;     // a launch holds the slot (on the GPU, or its results being copied out)
-    bool copying = false;  // the launch completed; the copier thread owns its results
+    bool copying = false;  // the launch completed; a copier thread owns its results
     std::vector<std::shared_ptr<Job>> jobs;
     std::vector<DevBuf<float>> pack_llr;    // per member, max_cw rows
     std::vector<DevBuf<uint8_t>> pack_out;  // per member, [info | ok | used] of one launch
@@ -137,7 +137,7 @@ public:
             throw;
         }
         thread_ = std::thread([this] { loop(); });
-        copier_ = std::thread([this] { copy_loop(); });
+        for (int i = 0; i < kCopierThreads; ++i) copiers_.emplace_back([this] { copy_loop(); });
     }
 
     ~Server()
@@ -160,7 +160,8 @@ public:
         cv_copy_.notify_all();
         std::lock_guard<std::mutex> jl(join_mu_);
         if (thread_.joinable()) thread_.join();
-        if (copier_.joinable()) copier_.join();
+        for (auto& c : copiers_)
+            if (c.joinable()) c.join();
     }
 
 private:
@@ -472,7 +473,7 @@ private:
 
     // Lock not held: polls the completion events of the launches still on the
     // GPU (polling_, taken under the lock) until one completes, a job arrives,
-    // the copier frees a slot, the batching deadline passes or wake_ is set.
+    // a copier frees a slot, the batching deadline passes or wake_ is set.
     void wait_event(uint64_t seen, uint64_t copies, uint64_t deadline)
     {
         for (;;) {
@@ -486,7 +487,7 @@ private:
         }
     }
 
-    // Completed launches go to the copier thread (lock held): the scheduler goes
+    // Completed launches go to the copier threads (lock held): the scheduler goes
     // straight on to admission and the next launches.
     bool harvest(std::unique_lock<std::mutex>&)
     {
@@ -874,11 +875,15 @@ private:
     static constexpr int kMinClass = 8, kClasses = 24;          // 256 B .. 2 GiB
     std::vector<std::vector<uint8_t>> spare_[kClasses];         // recycled result buffers by capacity class
     size_t spare_bytes_ = 0;
-    std::deque<Slot*> to_copy_;                                 // completed launches for the copier
+    std::deque<Slot*> to_copy_;                                 // completed launches for the copiers
     std::condition_variable cv_copy_;
-    std::atomic<uint64_t> copies_done_{0};                      // launches the copier has completed
+    std::atomic<uint64_t> copies_done_{0};                      // launches the copiers have completed
     std::vector<hipEvent_t> polling_;                           // scheduler: events of launches on the GPU
-    std::thread copier_;
+    // A launch's slot stays busy until its results are copied out (about 50 us of
+    // host memcpy per 65 536-codeword launch); two copiers take completed
+    // launches in parallel so that the copies do not queue behind each other.
+    static constexpr int kCopierThreads = 2;
+    std::vector<std::thread> copiers_;
     sicc_server_stats_t st_{};
     bool stop_ = false;
     bool hold_ = false;
