@@ -325,9 +325,10 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks, slots):
 
         # warm-up: the first launches on each slot allocate its packing and result
         # buffers (grow-only) and the first drive fills the recycled result-buffer
-        # pool; steady-state serving reuses them (two drives: in a process that has
-        # run the other legs first, the drive after the first still ran slower)
-        for _ in range(2):
+        # pool; steady-state serving reuses them (three drives: in a process that
+        # has run the other legs first, the copies into the recycled buffers still
+        # ran slower for two more drives)
+        for _ in range(3):
             srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         # peak: median of five all-at-once drives (one drive lasts tens of ms, so a
         # single stall would set every offered load below)

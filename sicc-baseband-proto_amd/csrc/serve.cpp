@@ -167,6 +167,7 @@ public:
 private:
     void setup(int slots, size_t budget)
     {
+        known_.reserve(size_t(1) << 16);  // no rehash while tens of thousands of jobs are open
         SICC_HIP(hipGetDevice(&device_));
         SICC_HIP(hipMalloc(reinterpret_cast<void**>(&base_), budget));
         SICC_HIP(hipStreamCreateWithFlags(&admit_stream_, hipStreamNonBlocking));
@@ -273,24 +274,39 @@ public:
 
     // Queue jobs under one lock acquisition (all or none: a duplicate id rejects
     // the group).
+    // Submission never takes the scheduler's lock (mu_): a client submitting in
+    // a tight loop re-took mu_ faster than the woken scheduler could, which kept
+    // the scheduler off the lock, and the GPU idle, for as long as the client
+    // kept submitting.  Ids go into known_ under ids_mu_, the jobs into the
+    // inbox under inbox_mu_ (held for the pointer pushes only), and the
+    // scheduler empties the inbox in admit().
     void enqueue(const std::shared_ptr<Job>* jobs, size_t count)
     {
         if (count == 0) return;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<std::mutex> lk(ids_mu_);
+            // one hash insert per job; a duplicate (already known, or twice in the
+            // group) takes the group's earlier inserts back out
             for (size_t i = 0; i < count; ++i) {
-                if (known_.count(jobs[i]->id)) throw std::invalid_argument("submit: job id already in use");
-                for (size_t q = 0; q < i; ++q)
-                    if (jobs[q]->id == jobs[i]->id) throw std::invalid_argument("submit: job id already in use");
+                if (!known_.emplace(jobs[i]->id, jobs[i]).second) {
+                    for (size_t q = 0; q < i; ++q) known_.erase(jobs[q]->id);
+                    throw std::invalid_argument("submit: job id already in use");
+                }
             }
-            for (size_t i = 0; i < count; ++i) {
-                known_[jobs[i]->id] = jobs[i];
-                arrivals_.push_back(jobs[i]);
-            }
-            st_.jobs_submitted += count;
-            arrived_.fetch_add(count, std::memory_order_release);
         }
-        cv_.notify_all();
+        {
+            std::lock_guard<std::mutex> lk(inbox_mu_);
+            for (size_t i = 0; i < count; ++i) inbox_.push_back(jobs[i]);
+            arrived_.fetch_add(count, std::memory_order_seq_cst);
+        }
+        // Only a scheduler asleep on cv_ needs the notify, and then under mu_:
+        // it sets sleeping_ before it tests for arrivals, so either that test
+        // sees this group or this load sees sleeping_ and the notify follows its
+        // wait (which holds mu_ until it blocks).
+        if (sleeping_.load(std::memory_order_seq_cst)) {
+            std::lock_guard<std::mutex> lk(mu_);
+            cv_.notify_all();
+        }
     }
 
     void submit(uint64_t id, int member, const float* llr, int count, bool dev)
@@ -314,7 +330,8 @@ public:
     {
         std::shared_ptr<Job> j;
         {
-            std::lock_guard<std::mutex> lk(mu_);
+            std::lock_guard<std::mutex> lk(mu_);  // t_done is written under mu_
+            std::lock_guard<std::mutex> il(ids_mu_);
             auto it = known_.find(id);
             if (it == known_.end() || it->second->t_done == 0) throw std::invalid_argument("fetch: job not completed");
             j = it->second;
@@ -366,15 +383,16 @@ public:
     void drain()
     {
         std::unique_lock<std::mutex> lk(mu_);
-        if (hold_ && st_.jobs_completed != st_.jobs_submitted)
+        if (hold_ && st_.jobs_completed != arrived_.load())
             throw std::invalid_argument("drain: the server is held with jobs pending");
-        cv_done_.wait(lk, [&] { return st_.jobs_completed == st_.jobs_submitted || failed_; });
+        cv_done_.wait(lk, [&] { return st_.jobs_completed == arrived_.load() || failed_; });
         if (failed_) throw std::runtime_error("server: " + fail_msg_);
     }
 
     double job_latency_and_forget(uint64_t id)
     {
         std::lock_guard<std::mutex> lk(mu_);
+        std::lock_guard<std::mutex> il(ids_mu_);
         auto it = known_.find(id);
         if (it == known_.end()) return -1.0;
         const double l = static_cast<double>(it->second->t_done - it->second->t_submit);
@@ -393,6 +411,7 @@ public:
     {
         std::lock_guard<std::mutex> lk(mu_);
         sicc_server_stats_t s = st_;
+        s.jobs_submitted = arrived_.load();
         s.pool_bytes = pool_.capacity();
         s.pool_high_water = pool_.high_water();
         s.mean_jobs_per_launch = st_.launches ? static_cast<double>(st_.jobs_completed) / st_.launches : 0.0;
@@ -440,11 +459,13 @@ private:
                 }
                 const bool inflight = std::any_of(slots_.begin(), slots_.end(), [](const Slot& s) { return s.busy; });
                 if (!progress) {
-                    if (inflight || (!hold_ && (!arrivals_.empty() || !resident_.empty()))) {
+                    if (inflight || (!hold_ && (pending_arrivals() || !resident_.empty()))) {
                         // Wait for a launch to finish, a new job, or the oldest
                         // resident job's batching deadline -- without the lock, so
                         // submitters are not starved by this thread's polling.
-                        const uint64_t seen = arrived_.load(std::memory_order_acquire);
+                        // jobs may arrive after admit() emptied the inbox: wake for
+                        // any job not yet taken (while held, for new ones only)
+                        const uint64_t seen = hold_ ? arrived_.load(std::memory_order_acquire) : moved_;
                         const uint64_t copies = copies_done_.load(std::memory_order_acquire);
                         polling_.clear();
                         for (auto& s : slots_)
@@ -459,7 +480,9 @@ private:
                         wait_event(seen, copies, deadline);
                         lk.lock();
                     } else {
-                        cv_.wait(lk, [&] { return stop_ || (!hold_ && (!arrivals_.empty() || !resident_.empty())); });
+                        sleeping_.store(true, std::memory_order_seq_cst);
+                        cv_.wait(lk, [&] { return stop_ || (!hold_ && (pending_arrivals() || !resident_.empty())); });
+                        sleeping_.store(false, std::memory_order_relaxed);
                     }
                 }
             }
@@ -598,9 +621,22 @@ private:
         }
     }
 
+    // Jobs submitted and not yet taken from the inbox, or taken and not admitted
+    // (lock held).
+    bool pending_arrivals() const
+    {
+        return !arrivals_.empty() || arrived_.load(std::memory_order_seq_cst) != moved_;
+    }
+
     bool admit()
     {
         bool any = false, copied = false;
+        if (arrived_.load(std::memory_order_acquire) != moved_) {
+            std::lock_guard<std::mutex> il(inbox_mu_);
+            moved_ += inbox_.size();
+            for (auto& j : inbox_) arrivals_.push_back(std::move(j));
+            inbox_.clear();
+        }
         while (!arrivals_.empty()) {
             auto j = arrivals_.front();
             const size_t bytes = footprint(j->member, j->count);
@@ -866,7 +902,12 @@ private:
     std::vector<BpBatch> fused_;              // issue's batches of one fused launch
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
-    std::deque<std::shared_ptr<Job>> arrivals_, resident_;
+    std::deque<std::shared_ptr<Job>> arrivals_, resident_;  // admission order (under mu_)
+    std::mutex inbox_mu_;                                   // guards inbox_ (taken after mu_)
+    std::deque<std::shared_ptr<Job>> inbox_;                // submitted, not yet taken by the scheduler
+    uint64_t moved_ = 0;                                    // jobs taken from the inbox (under mu_)
+    std::atomic<bool> sleeping_{false};                     // the scheduler waits on cv_ for arrivals
+    std::mutex ids_mu_;                                     // guards known_ (taken after mu_)
     std::deque<unsigned long long> completed_;
     static constexpr size_t kLogRecords = 65536;
     std::deque<sicc_server_launch_record_t> log_;  // completed launches, oldest first

@@ -172,6 +172,29 @@ def test_native_driver_latencies():
         d.close()
 
 
+def test_duplicate_job_id_rejected():
+    """A job id still known to the server is rejected; the first job keeps its
+    results and the id is free again once they are fetched."""
+    decs, codes = _members()
+    jobs = _jobs(codes, 2, 9)
+    with sa.Server(decs, budget_bytes=256 << 20, slots=2) as srv:
+        mem, llr = jobs[0]
+        srv.submit(7, mem, llr)
+        with pytest.raises(sa.SiccError, match="already in use"):
+            srv.submit(7, jobs[1][0], jobs[1][1])
+        srv.drain()
+        first = srv.fetch(7, llr.shape[0], codes[mem].k)
+        srv.submit(7, mem, llr)
+        srv.drain()
+        again = srv.fetch(7, llr.shape[0], codes[mem].k)
+        for a, b in zip(first, again):
+            if isinstance(a, np.ndarray):
+                np.testing.assert_array_equal(a, b)
+        assert srv.stats().jobs_completed == 2
+    for d in decs:
+        d.close()
+
+
 def test_ragged_info_width_round_trip():
     """k = 73 (neither a multiple of 8 nor of 64): results travel bit-packed from
     the GPU and are unpacked by fetch; every codeword equals the oracle."""
