@@ -19,11 +19,13 @@
  * that are idle when it is issued.
  * While launches are in flight the scheduler thread polls their completion
  * events (yielding between polls: one host core), so completions are seen
- * within microseconds; an idle server sleeps on a condition variable.  A
- * second thread copies each completed launch's results out of its pinned
- * buffer into the jobs' own buffers (recycled from fetched jobs) and completes
- * the jobs, so the scheduler keeps launching meanwhile; a job's latency runs
- * from its submission to the end of that copy.
+ * within microseconds; an idle server sleeps on a condition variable.  Two
+ * copier threads copy each completed launch's results out of its pinned
+ * buffer into the jobs' own buffers (recycled from fetched jobs) and complete
+ * the jobs, so the scheduler keeps launching meanwhile; a slot is free again
+ * once its copy is done, and a job's latency runs from its submission to the
+ * end of that copy.  Submission does not take the scheduler's lock: jobs wait
+ * in an inbox that the scheduler empties when it admits.
  * slots = 1 and max_jobs_per_launch = 1 give the plain one-job-per-launch server.
  * Results are identical to decoding each job alone (per-codeword decoding is
  * independent of batch composition).
