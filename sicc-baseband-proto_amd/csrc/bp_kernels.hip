@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 
 #include <cfloat>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -338,24 +339,34 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                 }
                 // smallest and second smallest |q| by a pairwise tree (depth log2 MAXDC
                 // instead of a MAXDC-long chain): pairs give (min, max); two (lo, second)
-                // pairs merge to (min lo, min(max lo, min second)).  Both are exact
+                // pairs merge to (min lo, min(max lo, min second)); an odd one out moves
+                // up a level as it is (a lone value's second is FLT_MAX).  Both are exact
                 // order statistics, capped at FLT_MAX as a sequential scan from
                 // (FLT_MAX, FLT_MAX) leaves them.
-                static_assert((MAXDC & (MAXDC - 1)) == 0, "row degree bound must be a power of two");
-                float lo[MAXDC / 2], se[MAXDC / 2];
+                constexpr int P = (MAXDC + 1) / 2;
+                float lo[P], se[P];
 #pragma unroll
                 for (int i = 0; i < MAXDC / 2; ++i) {
                     lo[i] = fminf(av[2 * i], av[2 * i + 1]);
                     se[i] = fmaxf(av[2 * i], av[2 * i + 1]);
                 }
+                if (MAXDC & 1) {
+                    lo[P - 1] = av[MAXDC - 1];
+                    se[P - 1] = FLT_MAX;
+                }
 #pragma unroll
-                for (int w = MAXDC / 2; w > 1; w >>= 1)
+                for (int w = P; w > 1; w = (w + 1) / 2) {
 #pragma unroll
                     for (int i = 0; i < w / 2; ++i) {
                         const float l = fminf(lo[2 * i], lo[2 * i + 1]);
                         se[i] = fminf(fmaxf(lo[2 * i], lo[2 * i + 1]), fminf(se[2 * i], se[2 * i + 1]));
                         lo[i] = l;
                     }
+                    if (w & 1) {
+                        lo[w / 2] = lo[w - 1];
+                        se[w / 2] = se[w - 1];
+                    }
+                }
                 const float m1 = fminf(lo[0], FLT_MAX);
                 const float m2 = fminf(se[0], FLT_MAX);
                 synw += __popcll(__ballot((xor_reduce(hb) >> 31) != 0u));
@@ -769,6 +780,35 @@ void launch_team(int team, int tpb, const BpCodeDev& code, const BpParams& p, co
     }
 }
 
+// Row-degree bound of the kernel instance for a code: its largest row degree
+// rounded up to 8, 10, 12, 14 or 16 (the check-node loop runs that many edge
+// slots per row, padding included, so a code whose rows reach 10 edges would do
+// 60 % more check-node work on the 16-slot instance).
+int dc_class(int max_dc) { return max_dc <= 8 ? 8 : max_dc <= 10 ? 10 : max_dc <= 12 ? 12 : max_dc <= 14 ? 14 : 16; }
+
+// Calls f(rows per thread, variables per thread, row-degree bound) with the three
+// as compile-time constants: (1, 2) for one row and two variables a thread, else
+// (2, 4).
+template <class F>
+void with_dc_class(int dc, bool small, F&& f)
+{
+    using std::integral_constant;
+    auto go = [&](auto maxdc) {
+        if (small)
+            f(integral_constant<int, 1>{}, integral_constant<int, 2>{}, maxdc);
+        else
+            f(integral_constant<int, 2>{}, integral_constant<int, 4>{}, maxdc);
+    };
+    switch (dc) {
+        case 8: return go(integral_constant<int, 8>{});
+        case 10: return go(integral_constant<int, 10>{});
+        case 12: return go(integral_constant<int, 12>{});
+        case 14: return go(integral_constant<int, 14>{});
+        case 16: return go(integral_constant<int, 16>{});
+        default: throw std::invalid_argument("BP row-degree class must be 8, 10, 12, 14 or 16");
+    }
+}
+
 // Rows and variables per thread for a team of `team` waves.
 void per_thread(const BpCodeDev& c, int team, int* mpl, int* npl)
 {
@@ -847,7 +887,7 @@ int bp_flood_instance(const BpCodeDev& c, const BpParams& p, const BpTune& tune)
     per_thread(c, team, &mpl, &npl);
     const bool small = mpl <= 1 && npl <= 2;
     const int tpb = team == 1 ? (tune.teams_per_block >= 4 ? 4 : (tune.teams_per_block >= 2 ? 2 : 1)) : 1;
-    return team << 8 | tpb << 4 | (small ? 4 : 0) | (c.max_dc <= 8 ? 2 : 0) | (p.damping != 0.0f ? 1 : 0);
+    return team << 12 | tpb << 8 | dc_class(c.max_dc) << 3 | (small ? 4 : 0) | (p.damping != 0.0f ? 1 : 0);
 }
 
 void launch_bp_flood_multi(const BpBatch* batches, int count, hipStream_t s)
@@ -861,23 +901,15 @@ void launch_bp_flood_multi(const BpBatch* batches, int count, hipStream_t s)
         if (batches[i].io.num_cw <= 0 || !batches[i].io.llr) throw std::invalid_argument("fused BP batch is empty");
         if (batches[i].io.work_ctr) throw std::invalid_argument("fused BP batches use static striding");
     }
-    const int team = key >> 8;
+    const int team = key >> 12;
     const bool small = (key & 4) != 0;
-    const bool dc8 = (key & 2) != 0;
+    const int dc = (key >> 3) & 31;
     for (int c0 = 0; c0 < count; c0 += kBpSegmentsPerLaunch) {
         const int c = count - c0 < kBpSegmentsPerLaunch ? count - c0 : kBpSegmentsPerLaunch;
         const BpBatch* b = batches + c0;
-        if (dc8) {
-            if (small)
-                launch_multi_team<1, 2, 8>(team, b, c, s);
-            else
-                launch_multi_team<2, 4, 8>(team, b, c, s);
-        } else {
-            if (small)
-                launch_multi_team<1, 2, 16>(team, b, c, s);
-            else
-                launch_multi_team<2, 4, 16>(team, b, c, s);
-        }
+        with_dc_class(dc, small, [&](auto mpl, auto npl, auto maxdc) {
+            launch_multi_team<decltype(mpl)::value, decltype(npl)::value, decltype(maxdc)::value>(team, b, c, s);
+        });
     }
 }
 
@@ -893,12 +925,10 @@ void launch_bp_flood(const BpCodeDev& code, const BpParams& p, const BpIo& io, i
     int mpl = 0, npl = 0;
     per_thread(code, team, &mpl, &npl);
     const bool small = mpl <= 1 && npl <= 2;
-    if (code.max_dc <= 8) {
-        if (small) return launch_team<1, 2, 8>(team, tpb, code, p, io, max_waves, s);
-        return launch_team<2, 4, 8>(team, tpb, code, p, io, max_waves, s);
-    }
-    if (small) return launch_team<1, 2, 16>(team, tpb, code, p, io, max_waves, s);
-    return launch_team<2, 4, 16>(team, tpb, code, p, io, max_waves, s);
+    with_dc_class(dc_class(code.max_dc), small, [&](auto mpl_c, auto npl_c, auto maxdc) {
+        launch_team<decltype(mpl_c)::value, decltype(npl_c)::value, decltype(maxdc)::value>(team, tpb, code, p, io,
+                                                                                           max_waves, s);
+    });
 }
 
 }  // namespace sicc
