@@ -33,6 +33,27 @@ namespace sicc {
 
 namespace {
 
+// Workgroup barrier that orders LDS only: global stores still in flight (this
+// thread's own check messages, which only this thread reads back) and global
+// loads issued ahead (the next colour's messages) are not waited for, as a full
+// __syncthreads() would.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Barrier after a step that changed the APP: LDS-only when the APP lives in LDS.
+template <bool APP_LDS>
+__device__ __forceinline__ void app_barrier()
+{
+    if (APP_LDS)
+        lds_barrier();
+    else
+        __syncthreads();
+}
+
 // 16 wavefronts per codeword; rows of degree above 16 keep 32 messages in
 // registers, which needs the larger register budget of 8 wavefronts.
 template <int MAXW>
@@ -41,9 +62,14 @@ constexpr int long_threads()
     return MAXW > 16 ? 512 : 1024;
 }
 
+// Rows of up to 8 edges with the APP in LDS: held to 64 VGPRs (a few registers
+// spill), so that two 1024-thread workgroups (eight waves per SIMD) share a CU:
+// N16200 layered 8.43 -> 6.33 ms per 4 096 frames with the next-colour loads
+// below.  With the APP in global memory the cap and those loads are slower
+// (N64800 32.7 -> 59.3 ms): every barrier there waits for all memory operations.
 template <int MAXW, bool LAYERED, bool APP_LDS>
-__global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCodeDev c, BpParams p, BpIo io,
-                                                                       BpLongScratch sc)
+__global__ __launch_bounds__(long_threads<MAXW>()) __attribute__((amdgpu_waves_per_eu(MAXW <= 8 && APP_LDS ? 8 : 1))) void
+bp_long_kernel(BpLongCodeDev c, BpParams p, BpIo io, BpLongScratch sc)
 {
     constexpr int kLongThreads = long_threads<MAXW>();
     extern __shared__ float lds_app[];
@@ -64,7 +90,37 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
         __syncthreads();
         int it = 0;
         bool ok = false;
+        // Column indices and old messages of one row (every slot of the group's
+        // width exists, padding included: the loads are issued together, not
+        // chained).
+        auto load_row = [&](int g, int pos, int* cv, float* rv) {
+            const int rows = c.grp_row[g + 1] - c.grp_row[g];
+            const int base = c.grp_edge[g];
+            const int width = c.grp_width[g];
+#pragma unroll
+            for (int j = 0; j < MAXW; ++j) {
+                if (j < width) {
+                    const int slot = base + j * rows + pos;
+                    cv[j] = c.edge_col[slot];
+                    rv[j] = r[slot];
+                } else {
+                    cv[j] = -1;
+                    rv[j] = 0.0f;
+                }
+            }
+        };
         while (it < p.max_iter) {
+            // Layered: a thread's first row of the next colour is loaded before the
+            // barrier that ends the current colour, so the loads overlap the wait.
+            // Its messages were last written by this thread (rows keep their thread
+            // from iteration to iteration) and the column indices are constant, so
+            // only the APP reads need the barrier.  (Rows of more than 16 edges keep
+            // one row in registers; with the APP in global memory the barrier waits
+            // for the loads, so they are issued after it.)
+            constexpr bool PF = LAYERED && APP_LDS && MAXW <= 16;
+            int pcv[MAXW];
+            float prv[MAXW];
+            if (PF && tid < c.grp_row[1] - c.grp_row[0]) load_row(0, tid, pcv, prv);
             for (int g = 0; g < c.groups; ++g) {
                 const int rows = c.grp_row[g + 1] - c.grp_row[g];
                 const int base = c.grp_edge[g];
@@ -74,17 +130,14 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
                     int cv[MAXW];
                     float m1 = FLT_MAX, m2 = FLT_MAX;
                     int mi = -1, neg = 0, deg = 0;
-                    // every slot of the group's width exists (padding included): the
-                    // column and message loads are issued together, not chained
+                    if (PF && pos == tid) {
 #pragma unroll
-                    for (int j = 0; j < MAXW; ++j) {
-                        if (j < width) {
-                            const int slot = base + j * rows + pos;
-                            cv[j] = c.edge_col[slot];
-                            rv[j] = r[slot];
-                        } else {
-                            cv[j] = -1;
+                        for (int j = 0; j < MAXW; ++j) {
+                            cv[j] = pcv[j];
+                            rv[j] = prv[j];
                         }
+                    } else {
+                        load_row(g, pos, cv, rv);
                     }
 #pragma unroll
                     for (int j = 0; j < MAXW; ++j) {
@@ -127,7 +180,13 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
                         }
                     }
                 }
-                __syncthreads();
+                if (PF && g + 1 < c.groups && tid < c.grp_row[g + 2] - c.grp_row[g + 1]) load_row(g + 1, tid, pcv, prv);
+                // layered: the next colour reads this one's APP updates; flooding:
+                // the column sums read other threads' messages from global memory
+                if (LAYERED)
+                    app_barrier<APP_LDS>();
+                else
+                    __syncthreads();
             }
             if (!LAYERED) {
                 for (int v = tid; v < c.n; v += kLongThreads) {
@@ -162,7 +221,7 @@ __global__ __launch_bounds__(long_threads<MAXW>()) void bp_long_kernel(BpLongCod
                 }
             }
             if (tid == 0) s_flag[fl ^ 1] = 0;
-            __syncthreads();
+            app_barrier<APP_LDS>();
             ok = s_flag[fl] == 0;
             ++it;
             if (ok) break;
