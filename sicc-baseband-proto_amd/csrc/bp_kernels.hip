@@ -40,13 +40,24 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroup barrier ordering LDS only.  A multi-wave team exchanges nothing
+// through global memory inside the decode loop, so its barriers need not wait
+// for the next codeword's LLR loads or this codeword's output stores, as a
+// full __syncthreads() would.
+__device__ __forceinline__ void group_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int TEAM>
 __device__ __forceinline__ void team_sync()
 {
     if (TEAM == 1)
         wave_lds_sync();
     else
-        __syncthreads();
+        group_lds_sync();
 }
 
 __global__ void zero_i32_kernel(int* p, int count)
@@ -240,7 +251,7 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
             b = __builtin_amdgcn_readlane(b, 0);
         } else {
             if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
-            __syncthreads();
+            group_lds_sync();
             b = red[4];
         }
         cw = b;
@@ -378,7 +389,7 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                 // team-wide syndrome weight through an LDS counter, double-buffered by iteration
                 int* ctr = red + (it & 1);
                 if (lane == 0 && synw) atomicAdd(ctr, synw);
-                __syncthreads();
+                group_lds_sync();
                 synw = *ctr;
                 if (tid == 0) red[(it + 1) & 1] = 0;
             }
@@ -463,7 +474,7 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
         }
         if (TEAM > 1) {
             // the counter the last check pass used must read zero for the next codeword
-            __syncthreads();
+            group_lds_sync();
             if (tid == 0) red[0] = red[1] = 0;
         }
 
@@ -515,7 +526,7 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                 slot = __builtin_amdgcn_readlane(slot, 0);
             } else {
                 if (tid == 0) red[2] = atomicAdd(io.osd_count, 1);
-                __syncthreads();
+                group_lds_sync();
                 slot = red[2];
             }
             float* dst = io.osd_llr + static_cast<size_t>(slot) * n;
@@ -538,9 +549,9 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
             int b = 0;
             {
                 if (tid == 0) red[4] = atomicAdd(io.work_ctr, chunk);
-                __syncthreads();
+                group_lds_sync();
                 b = red[4];
-                __syncthreads();
+                group_lds_sync();
             }
             cw = b;
             lim = min(b + chunk, io.num_cw);
