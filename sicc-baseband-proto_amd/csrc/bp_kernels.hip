@@ -573,9 +573,21 @@ __device__ __forceinline__ int grid_team() { return static_cast<int>(blockIdx.x 
 template <int T>
 __device__ __forceinline__ int grid_teams() { return static_cast<int>(gridDim.x * (blockDim.x / T)); }
 
+// Teams of 8 and 16 waves with one row a thread are held to 64 VGPRs (eight
+// waves per SIMD, two workgroups per CU) up to the row-degree classes where the
+// few spilled registers cost less than the second workgroup gains: 16-wave
+// teams through 14 edges, 8-wave teams through 12 (32 768 codewords, n = 2048
+// rate 11/16: 2.29 -> 1.65 ms, rate 3/4: 2.22 -> 2.00 ms; n = 1024 rate 1/2:
+// 0.94 -> 0.85 ms; the 8-wave, 14-edge instance was slower capped).
+template <int MPL, int MAXDC, int TEAM>
+constexpr int team_waves_per_eu()
+{
+    return MPL == 1 && ((TEAM == 16 && MAXDC <= 14) || (TEAM == 8 && MAXDC <= 12)) ? 8 : 1;
+}
+
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
-__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io,
-                                                                              int e_pad)
+__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) __attribute__((amdgpu_waves_per_eu(
+    team_waves_per_eu<MPL, MAXDC, TEAM>()))) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
     bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(code, p, io, e_pad, grid_team<64 * TEAM>(), grid_teams<64 * TEAM>());
 }
@@ -599,7 +611,8 @@ struct BpSegmentTable {
 };
 
 template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
-__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) void bp_flood_multi_kernel(BpSegmentTable t)
+__global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) __attribute__((amdgpu_waves_per_eu(
+    team_waves_per_eu<MPL, MAXDC, TEAM>()))) void bp_flood_multi_kernel(BpSegmentTable t)
 {
     constexpr int T = 64 * TEAM;
     int si = 0;
