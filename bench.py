@@ -309,15 +309,15 @@ def slo_point(loads):
     return best
 
 
-def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks, slots):
+def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks, slots, max_cw, min_cw, max_wait_us):
     """Peak (all jobs at t = 0), then Poisson arrivals at fractions of the peak,
     then the plain one-job-per-launch server on the same jobs."""
     n_jobs = len(members)
     bits = float(sum(int(c) * ks[m] for m, c in zip(members, counts)))
-    out = {"jobs": n_jobs, "batching": {"slots": slots, "min_launch_codewords": a.serve_min_cw,
-                                        "max_wait_us": a.serve_max_wait_us}}
-    with sa.Server(decoders, budget_bytes=16 << 30, slots=slots, max_launch_codewords=1 << 16) as srv:
-        srv.set_batching(a.serve_min_cw, a.serve_max_wait_us)
+    out = {"jobs": n_jobs, "batching": {"slots": slots, "max_launch_codewords": max_cw,
+                                        "min_launch_codewords": min_cw, "max_wait_us": max_wait_us}}
+    with sa.Server(decoders, budget_bytes=16 << 30, slots=slots, max_launch_codewords=max_cw) as srv:
+        srv.set_batching(min_cw, max_wait_us)
 
         def jobs_per_launch(s0, s1):
             dl = s1.launches - s0.launches
@@ -400,7 +400,8 @@ def leg_served(a, sa, np, torch, dec128):
     out = {"ccsds128": {"job_codewords": "uniform 1..512", "code": "CCSDS (128,64) BP+OSD order-1",
                         "ebn0_db": a.ebn0}}
     torch.cuda.synchronize()
-    out["ccsds128"].update(serve_sweep(a, sa, np, rng, [dec128], [0] * n_jobs, counts, ptrs, [64], a.serve_slots))
+    out["ccsds128"].update(serve_sweep(a, sa, np, rng, [dec128], [0] * n_jobs, counts, ptrs, [64], a.serve_slots,
+                                                  1 << 16, a.serve_min_cw, a.serve_max_wait_us))
     # the 15-code QC mix at FER ~1e-3: one flooding BP decoder per code, LLR pools in HBM
     shapes = sorted(MIX_EBN0_SERVE)
     decs, pools, ks = [], [], []
@@ -424,7 +425,8 @@ def leg_served(a, sa, np, torch, dec128):
                   "code": "QC-LDPC mix n 128..2048, rates 1/2, 11/16, 3/4, flooding BP 50 it",
                   "ebn0_db": {f"n{n}_mb{mb}": e for (n, mb), e in sorted(MIX_EBN0_SERVE.items())}}
     out["mix"]["tunes"] = [d.get_tune() for d in decs]
-    out["mix"].update(serve_sweep(a, sa, np, rng, decs, [int(x) for x in mem], cnt, mptr, ks, a.serve_mix_slots))
+    out["mix"].update(serve_sweep(a, sa, np, rng, decs, [int(x) for x in mem], cnt, mptr, ks, a.serve_mix_slots,
+                                             a.serve_mix_max_cw, a.serve_mix_min_cw, a.serve_mix_max_wait_us))
     for d in decs:
         d.close()
     return out
@@ -854,6 +856,12 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=12,
                     help="(128,64) server: launches in flight at once (each with its own stream and results "
                          "buffer); more slots overlap one launch's BP tail with the next launches")
+    ap.add_argument("--serve-mix-max-cw", type=int, default=1 << 16,
+                    help="15-code mix server: largest launch in codewords")
+    ap.add_argument("--serve-mix-min-cw", type=int, default=8192,
+                    help="15-code mix server: resident codewords that start a launch while another is in flight")
+    ap.add_argument("--serve-mix-max-wait-us", type=float, default=200.0,
+                    help="15-code mix server: oldest resident job's wait that starts a launch anyway")
     ap.add_argument("--serve-mix-slots", type=int, default=4,
                     help="15-code mix server: launches in flight at once; a mix launch already spreads its "
                          "code groups over the idle slots' streams, and slots beyond the four hardware "
