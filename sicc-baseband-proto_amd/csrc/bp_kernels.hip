@@ -321,24 +321,59 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
         bool ok = false;
         int it = 0;
         for (;;) {
-            // ---- check-node gather: q, min1 / min2 / sign, and the syndrome ----
+            // ---- check-node gather and the syndrome of the current APP ----
+            // The syndrome decides first: a pass that ends the decode (zero syndrome,
+            // or the last iteration) never needs q or the row minima, which are most
+            // of a row's check-node instructions.
             int synw = 0;
+            float ga[MPL][MAXDC];
+#pragma unroll
+            for (int s = 0; s < MPL; ++s) {
+                unsigned hb[MAXDC];
+#pragma unroll
+                for (int j = 0; j < MAXDC; ++j) {
+                    ga[s][j] = app[em[s][j] >> 16];
+                    hb[j] = __float_as_uint(ga[s][j]);  // canonical APP: sign bit == (APP < 0)
+                }
+                synw += __popcll(__ballot((xor_reduce(hb) >> 31) != 0u));
+            }
+            if (TEAM > 1) {
+                // team-wide syndrome weight through an LDS counter, double-buffered by iteration
+                int* ctr = red + (it & 1);
+                if (lane == 0 && synw) atomicAdd(ctr, synw);
+                group_lds_sync();
+                synw = *ctr;
+                if (tid == 0) red[(it + 1) & 1] = 0;
+            }
+            if (it > 0) {
+                if (synw < best_w || synw == 0) {
+                    best_w = synw;
+#pragma unroll
+                    for (int t = 0; t < NPL; ++t) bestr[t] = appr[t];
+                }
+                if (synw == 0) {
+                    ok = true;
+                    break;  // the messages of this pass would never be used
+                }
+            } else {
+                best_w = synw;
+            }
+            if (it == p.max_iter) break;
+            // ---- q, min1 / min2 and the sign product of every row ----
             float qv[MPL][MAXDC];
             float m1s[MPL], m2s[MPL];
             unsigned sps[MPL];
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
-                unsigned hb[MAXDC], qb[MAXDC];
+                unsigned qb[MAXDC];
                 float av[MAXDC];
 #pragma unroll
                 for (int j = 0; j < MAXDC; ++j) {
-                    const float a = app[em[s][j] >> 16];
-                    hb[j] = __float_as_uint(a);  // canonical APP: sign bit == (APP < 0)
                     // The LDS APP is never -0, so a - r_old is never -0 either (x - y is
                     // -0 only for x = -0, y = +0): q's sign bit is (q < 0) as in the
                     // reference, whose q can differ only in the sign of a zero.  Damped
                     // q can underflow to -0 and is canonicalized.
-                    float q = a - rold[s][j];
+                    float q = ga[s][j] - rold[s][j];
                     if (DAMP) {
                         if (it > 0) q = (1.0f - damp) * q + damp * qold[s][j];
                         qold[s][j] = q;
@@ -378,35 +413,10 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                         se[w / 2] = se[w - 1];
                     }
                 }
-                const float m1 = fminf(lo[0], FLT_MAX);
-                const float m2 = fminf(se[0], FLT_MAX);
-                synw += __popcll(__ballot((xor_reduce(hb) >> 31) != 0u));
-                m1s[s] = m1;
-                m2s[s] = m2;
+                m1s[s] = fminf(lo[0], FLT_MAX);
+                m2s[s] = fminf(se[0], FLT_MAX);
                 sps[s] = xor_reduce(qb);
             }
-            if (TEAM > 1) {
-                // team-wide syndrome weight through an LDS counter, double-buffered by iteration
-                int* ctr = red + (it & 1);
-                if (lane == 0 && synw) atomicAdd(ctr, synw);
-                group_lds_sync();
-                synw = *ctr;
-                if (tid == 0) red[(it + 1) & 1] = 0;
-            }
-            if (it > 0) {
-                if (synw < best_w || synw == 0) {
-                    best_w = synw;
-#pragma unroll
-                    for (int t = 0; t < NPL; ++t) bestr[t] = appr[t];
-                }
-                if (synw == 0) {
-                    ok = true;
-                    break;  // the messages of this pass would never be used
-                }
-            } else {
-                best_w = synw;
-            }
-            if (it == p.max_iter) break;
             // ---- check-node messages ----
 #pragma unroll
             for (int s = 0; s < MPL; ++s) {
