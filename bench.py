@@ -439,7 +439,7 @@ def leg_mixed(a, sa, np, torch, stream):
     F = a.mixed_frames
     shapes = sorted(MIX_EBN0)
     draw = rng.integers(0, len(shapes), F)
-    decs, groups, truth, keep = [], [], [], []
+    decs, groups, truth, keep, cpu_llr = [], [], [], [], []
     info_bits_total = 0
     for i, (n, mb) in enumerate(shapes):
         m, rp, ci = sa.qc_code(n, mb, 3, seed=1000 + n + mb)
@@ -456,6 +456,8 @@ def leg_mixed(a, sa, np, torch, stream):
         groups.append(sa.Group(i, cnt, d_llr.data_ptr(), d_info.data_ptr(), d_ok.data_ptr(), d_it.data_ptr(), None))
         truth.append(info)
         keep.append((d_llr, d_info, d_ok, d_it))
+        # the CPU baseline's sample: each code's share of the first frames, in proportion
+        cpu_llr.append((n, m, rp, ci, llr[: min(cnt, -(-a.cpu_mixed_frames * cnt // F))]))
         info_bits_total += cnt * dec.k
     md = sa.MultiDecoder(decs)
     sh = stream.cuda_stream
@@ -483,8 +485,10 @@ def leg_mixed(a, sa, np, torch, stream):
     torch.cuda.synchronize()
     per_code = {}
     fe_total = 0
+    gots = []
     for (n, mb), info, (d_llr, d_info, d_ok, d_it), dec in zip(shapes, truth, keep, decs):
         got = d_info.cpu().numpy()
+        gots.append(got)
         fe = int((got != info).any(axis=1).sum())
         fe_total += fe
         per_code[f"n{n}_k{dec.k}"] = {"frames": int(info.shape[0]), "ebn0_db": MIX_EBN0[(n, mb)],
@@ -494,11 +498,43 @@ def leg_mixed(a, sa, np, torch, stream):
     md.close()
     for d in decs:
         d.close()
-    return {"frames": F, "codes": len(shapes), "ms_per_batch": round(ms, 4),
-            "value": round(info_bits_total / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
-            "fer": round(fe_total / F, 5), "graph": True,
-            "sum_alone_ms": round(sum(alone), 4), "work_sum_bound_ms": round(sum(bound), 4),
-            "makespan_over_bound": round(ms / sum(bound), 4), "per_code": per_code}
+    out = {"frames": F, "codes": len(shapes), "ms_per_batch": round(ms, 4),
+           "value": round(info_bits_total / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
+           "fer": round(fe_total / F, 5), "graph": True,
+           "sum_alone_ms": round(sum(alone), 4), "work_sum_bound_ms": round(sum(bound), 4),
+           "makespan_over_bound": round(ms / sum(bound), 4), "per_code": per_code}
+    if a.cpu_mixed_frames > 0:
+        out["cpu_baseline"] = cpu_mixed(np, cpu_llr, gots)
+    return out
+
+
+def cpu_mixed(np, cpu_llr, gots):
+    """The mixed batch's first frames (each code's share in proportion) through
+    the oracle's flooding min-sum (LINK:3013-3094, 50 iterations, alpha 0.75),
+    code by code, one thread and the host's CPU share; agreement with the GPU's
+    information bits on the sample is reported."""
+    import oracle
+
+    codes = [oracle.Code(n, m, rp, ci, True) for n, m, rp, ci, _ in cpu_llr]
+    gens = [c.generator() for c in codes]  # code setup stays out of the timed decode
+
+    def run(threads):
+        t0 = time.perf_counter()
+        infos = []
+        for code, gen, (_, _, _, _, llr) in zip(codes, gens, cpu_llr):
+            infos.append(code.bposd(llr, gen=gen, max_iter=50, alpha=0.75, order=-1, threads=threads)["info"])
+        return time.perf_counter() - t0, infos
+
+    t1, infos = run(1)
+    T = host_threads()
+    tm, _ = run(T)
+    frames = sum(x[4].shape[0] for x in cpu_llr)
+    bits = sum(int(inf.size) for inf in infos)
+    agree = all(np.array_equal(inf, got[: inf.shape[0]]) for inf, got in zip(infos, gots))
+    return {"value": round(bits / t1 / 1e6, 3), "unit": "info Mbps", "cores": 1, "kind": "port",
+            "sample": f"first {frames} frames of the batch (each code's share), oracle min-sum in C per code",
+            "all_cores": {"value": round(bits / tm / 1e6, 3), "cores": T, "nproc": os.cpu_count()},
+            "agrees_with_gpu": bool(agree)}
 
 
 # --------------------------------------------------------------------- ofdm --
@@ -868,6 +904,8 @@ def main() -> int:
                          "queues only add queueing")
     ap.add_argument("--mixed-frames", type=int, default=65536)
     ap.add_argument("--mixed-reps", type=int, default=5)
+    ap.add_argument("--cpu-mixed-frames", type=int, default=8000,
+                    help="frames of the mixed batch through the CPU oracle (0: skip)")
     ap.add_argument("--ofdm-frames", type=int, default=4096)
     ap.add_argument("--ofdm-reps", type=int, default=5)
     ap.add_argument("--cpu-ofdm-frames", type=int, default=24,
