@@ -716,6 +716,8 @@ def leg_long(a, sa, np, torch, stream):
         cw = code.encode(info)
         llr = oracle.awgn_llr(cw, ebn0, k / n, rng)
         d_llr = torch.from_numpy(llr).to(dev)
+        S = min(F, max(a.cpu_long_frames * F // max(a.long_frames, 1), 1)) if a.cpu_long_frames > 0 else 0
+        llr_cpu = llr[:S].copy()
         del llr, cw
         d_info = torch.empty((F, k), dtype=torch.uint8, device=dev)
         d_ok = torch.empty(F, dtype=torch.uint8, device=dev)
@@ -735,18 +737,51 @@ def leg_long(a, sa, np, torch, stream):
         dec.set_timing(False)
         kms = kt["bp_ms"] / max(kt["bp_launches"], 1)
         iters = d_it.cpu().numpy()
-        fe = int((d_info.cpu().numpy() != info).any(axis=1).sum())
+        got = d_info.cpu().numpy()
+        fe = int((got != info).any(axis=1).sum())
         E = int(rp[-1])
+        cpu = cpu_long(np, code, llr_cpu, got) if S > 0 else None
         pts.append({"code": name.replace(".alist", ""), "n": n, "k": k, "edges": E, "frames": F, "ebn0_db": ebn0,
                     "schedule": "layered (greedy colouring)", "alpha": 0.8, "offset": 0.15, "damping": 0.15,
                     "fer": round(fe / F, 5), "mean_iterations": round(float(iters.mean()), 2),
                     "ms": round(ms, 3), "kernel_ms": round(kms, 3),
                     "value": round(F * k / (ms * 1e-3) / 1e6, 2), "unit": "info Mbps",
                     "codewords_per_s": round(F / (ms * 1e-3), 1),
-                    "edge_updates_per_s": round(float(iters.sum()) * E / (kms * 1e-3), 1)})
+                    "edge_updates_per_s": round(float(iters.sum()) * E / (kms * 1e-3), 1),
+                    "cpu_baseline": cpu})
         dec.close()
         del d_llr, d_info
     return {"points": pts}
+
+
+def cpu_long(np, code, llr, got):
+    """The long-code leg's first frames through the oracle's layered min-sum
+    (greedy colouring order, the same normalization, offset and damping), one
+    frame after another on one thread, and over a thread pool of the host's CPU
+    share; agreement with the GPU's information bits is reported."""
+    import concurrent.futures as cf
+
+    order = code.layer_order()[0]  # colouring stays out of the timed decode
+    cols = np.asarray(code.info_cols)
+
+    def one(f):
+        r = code.nms_layered(llr[f], order=order, max_iter=50, alpha=0.8, offset=0.15, damping=0.15)
+        return r["hard"][cols]
+
+    S = llr.shape[0]
+    t0 = time.perf_counter()
+    infos = [one(f) for f in range(S)]
+    t1 = time.perf_counter() - t0
+    T = host_threads()
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(T) as ex:
+        list(ex.map(one, range(S)))
+    tm = time.perf_counter() - t0
+    bits = S * code.k
+    return {"value": round(bits / t1 / 1e6, 3), "unit": "info Mbps", "cores": 1, "kind": "port",
+            "sample": f"first {S} frames of the point, oracle layered min-sum in C",
+            "all_cores": {"value": round(bits / tm / 1e6, 3), "cores": T, "nproc": os.cpu_count()},
+            "agrees_with_gpu": bool(all(np.array_equal(infos[f], got[f]) for f in range(S)))}
 
 
 # ------------------------------------------------------------------- cpu ----
@@ -915,6 +950,8 @@ def main() -> int:
     ap.add_argument("--osd-frames", type=int, default=262144)
     ap.add_argument("--long-frames", type=int, default=4096)
     ap.add_argument("--long-reps", type=int, default=3)
+    ap.add_argument("--cpu-long-frames", type=int, default=64,
+                    help="N16200 frames through the CPU oracle (N64800: a quarter as many; 0: skip)")
     ap.add_argument("--legs", default="cfg1,served,mixed,ofdm,osd,long,cpu",
                     help="comma list of extra legs: cfg1, served, mixed, ofdm, osd, long, cpu ('' = headline only)")
     ap.add_argument("--no-cpu", action="store_true")
