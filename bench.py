@@ -323,11 +323,10 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks, slots, max_
             dl = s1.launches - s0.launches
             return round((s1.jobs_completed - s0.jobs_completed) / dl, 2) if dl else None
 
-        # warm-up: the first launches on each slot allocate its packing and result
-        # buffers (grow-only) and the first drive fills the recycled result-buffer
-        # pool; steady-state serving reuses them (three drives: in a process that
-        # has run the other legs first, the copies into the recycled buffers still
-        # ran slower for two more drives)
+        # warm-up: the first drive fills the server's recycled result-buffer pool
+        # (fresh heap pages before that); steady-state serving reuses it (three
+        # drives: in a process that has run the other legs first, the copies into
+        # the recycled buffers still ran slower for two more drives)
         for _ in range(3):
             srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         # peak: median of five all-at-once drives (one drive lasts tens of ms, so a
