@@ -172,6 +172,49 @@ __device__ __forceinline__ unsigned xor_reduce(const unsigned (&v)[N])
     return x;
 }
 
+// Smallest and second smallest (multiset order statistics: a repeated minimum
+// is also the second) of N floats, none NaN.  Groups of three give
+// (min3, med3), a pair (min, max), a lone value (v, FLT_MAX); the groups'
+// minima recurse the same way, and the second smallest overall is
+// min(second smallest of the minima, smallest of the seconds): for the group
+// i holding the overall minimum, every other element is either another
+// group's minimum or at least it, or a value of group i at least its second.
+// For 8 values this is 10 v_min3 / v_med3 / v_min / v_max instructions in two
+// dependent levels, against 18 in four for a pairwise tree.
+template <int N>
+__device__ __forceinline__ void min_two(const float* v, float& lo, float& se)
+{
+    constexpr int G = (N + 2) / 3;
+    float gl[G], gs[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int b = 3 * g;
+        const int c = N - b < 3 ? N - b : 3;
+        if (c == 3) {
+            gl[g] = fminf(fminf(v[b], v[b + 1]), v[b + 2]);
+            gs[g] = __builtin_amdgcn_fmed3f(v[b], v[b + 1], v[b + 2]);
+        } else if (c == 2) {
+            gl[g] = fminf(v[b], v[b + 1]);
+            gs[g] = fmaxf(v[b], v[b + 1]);
+        } else {
+            gl[g] = v[b];
+            gs[g] = FLT_MAX;
+        }
+    }
+    if constexpr (G == 1) {
+        lo = gl[0];
+        se = gs[0];
+    } else {
+        float ll, ls;
+        min_two<G>(gl, ll, ls);
+        float sm = gs[0];
+#pragma unroll
+        for (int g = 1; g < G; ++g) sm = fminf(sm, gs[g]);
+        lo = ll;
+        se = fminf(ls, sm);
+    }
+}
+
 // a + r[0] + r[1] + ... + r[D-1], summed in that order, with every LDS read
 // issued before the first add (immediate offsets: no per-term wait or address
 // arithmetic).
@@ -188,7 +231,9 @@ __device__ __forceinline__ float col_sum(float a, const float* r)
 
 // One team's decode loop: team `gteam` of `nteams` takes codewords gteam,
 // gteam + nteams, ... (or chunks from io.work_ctr).
-template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+// OFF = false: a build for launches whose offset is zero, where
+// alpha * max(m - 0, 0) is alpha * m for every m >= +0.
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM, bool OFF = true>
 __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo io, int e_pad, int gteam, int nteams)
 {
     extern __shared__ float lds[];
@@ -383,38 +428,13 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                     qb[j] = __float_as_uint(q);
                     av[j] = fabsf(q);
                 }
-                // smallest and second smallest |q| by a pairwise tree (depth log2 MAXDC
-                // instead of a MAXDC-long chain): pairs give (min, max); two (lo, second)
-                // pairs merge to (min lo, min(max lo, min second)); an odd one out moves
-                // up a level as it is (a lone value's second is FLT_MAX).  Both are exact
-                // order statistics, capped at FLT_MAX as a sequential scan from
-                // (FLT_MAX, FLT_MAX) leaves them.
-                constexpr int P = (MAXDC + 1) / 2;
-                float lo[P], se[P];
-#pragma unroll
-                for (int i = 0; i < MAXDC / 2; ++i) {
-                    lo[i] = fminf(av[2 * i], av[2 * i + 1]);
-                    se[i] = fmaxf(av[2 * i], av[2 * i + 1]);
-                }
-                if (MAXDC & 1) {
-                    lo[P - 1] = av[MAXDC - 1];
-                    se[P - 1] = FLT_MAX;
-                }
-#pragma unroll
-                for (int w = P; w > 1; w = (w + 1) / 2) {
-#pragma unroll
-                    for (int i = 0; i < w / 2; ++i) {
-                        const float l = fminf(lo[2 * i], lo[2 * i + 1]);
-                        se[i] = fminf(fmaxf(lo[2 * i], lo[2 * i + 1]), fminf(se[2 * i], se[2 * i + 1]));
-                        lo[i] = l;
-                    }
-                    if (w & 1) {
-                        lo[w / 2] = lo[w - 1];
-                        se[w / 2] = se[w - 1];
-                    }
-                }
-                m1s[s] = fminf(lo[0], FLT_MAX);
-                m2s[s] = fminf(se[0], FLT_MAX);
+                // smallest and second smallest |q|: exact order statistics, capped at
+                // FLT_MAX as a sequential scan from (FLT_MAX, FLT_MAX) leaves them (a
+                // padded edge's |q| can overflow to +inf)
+                float lo, se;
+                min_two<MAXDC>(av, lo, se);
+                m1s[s] = fminf(lo, FLT_MAX);
+                m2s[s] = fminf(se, FLT_MAX);
                 sps[s] = xor_reduce(qb);
             }
             // ---- check-node messages ----
@@ -425,8 +445,8 @@ __device__ __forceinline__ void bp_flood_body(BpCodeDev code, BpParams p, BpIo i
                 const float m1 = m1s[s];
                 // with off == 0, m - 0 == m and max(m, 0) == m for every m >= +0, so
                 // the offset needs no test
-                const float a1 = alpha * fmaxf(m1 - off, 0.0f);
-                const float a2 = alpha * fmaxf(m2s[s] - off, 0.0f);
+                const float a1 = OFF ? alpha * fmaxf(m1 - off, 0.0f) : alpha * m1;
+                const float a2 = OFF ? alpha * fmaxf(m2s[s] - off, 0.0f) : alpha * m2s[s];
                 // keep the two products per row (not one per edge after the select)
                 float b1 = a1, b2 = a2;
                 asm volatile("" : "+v"(b1), "+v"(b2));
@@ -658,7 +678,7 @@ struct BpIoLean {
 // STEAL: the build with the work-stealing cursor needs a few more registers;
 // it is held to eight waves per SIMD (a 12-byte spill), the static build is not
 // constrained (62 registers, no spill).
-template <bool STEAL>
+template <bool STEAL, bool OFF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEAL ? 8 : 1))) void bp_flood_kernel_lean(
     BpCodeDev code, BpParams p, BpIoLean l, int e_pad)
 {
@@ -674,7 +694,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEAL ? 8 :
     io.osd_count = l.osd_count;
     io.work_ctr = STEAL ? l.work_ctr : nullptr;
     io.chunk = l.chunk;
-    bp_flood_body<1, 2, 8, false, 1>(code, p, io, e_pad, grid_team<64>(), grid_teams<64>());
+    bp_flood_body<1, 2, 8, false, 1, OFF>(code, p, io, e_pad, grid_team<64>(), grid_teams<64>());
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bp_flood_kernel_w8(BpCodeDev code,
@@ -706,9 +726,10 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
             kern = bp_flood_kernel_w8;
         }
     }
-    const void* lean_fn = io.work_ctr ? reinterpret_cast<const void*>(bp_flood_kernel_lean<true>)
-                                      : reinterpret_cast<const void*>(bp_flood_kernel_lean<false>);
-    const void* kfn = lean ? lean_fn : reinterpret_cast<const void*>(kern);
+    const bool off = p.offset != 0.0f;
+    auto lean_kern = io.work_ctr ? (off ? bp_flood_kernel_lean<true, true> : bp_flood_kernel_lean<true, false>)
+                                 : (off ? bp_flood_kernel_lean<false, true> : bp_flood_kernel_lean<false, false>);
+    const void* kfn = lean ? reinterpret_cast<const void*>(lean_kern) : reinterpret_cast<const void*>(kern);
     // A persistent grid larger than what the GPU holds at once would leave its
     // last teams to start after the first ones finished all their work.
     const long resident_teams = static_cast<long>(resident_blocks_per_cu(kfn, T * tpb, lds)) * device_cu_count() * tpb;
@@ -719,10 +740,7 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     if (lean) {
         const BpIoLean l{io.llr,     io.num_cw,   io.chunk,     io.info_bits, io.success, io.iterations,
                          io.used_osd, io.osd_llr, io.osd_index, io.osd_count, io.work_ctr};
-        if (io.work_ctr)
-            hipLaunchKernelGGL(bp_flood_kernel_lean<true>, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
-        else
-            hipLaunchKernelGGL(bp_flood_kernel_lean<false>, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
+        hipLaunchKernelGGL(lean_kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, l, e_pad);
     } else {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(T * tpb), lds, s, code, p, io, e_pad);
     }

@@ -44,14 +44,16 @@ def test_bp_bit_exact(ccsds, ebn0):
     assert not g["used_osd"].any()
 
 
+@pytest.mark.parametrize("offset", [0.0, 0.15])
 @pytest.mark.parametrize("osd_order", [-1, 1])
-def test_hard_output_launch_matches_oracle(ccsds, osd_order):
+def test_hard_output_launch_matches_oracle(ccsds, osd_order, offset):
     # Device decode asking only for hard outputs (information bits, flags,
-    # iteration counts) runs the lean single-wave BP kernel; it must agree with
-    # the oracle exactly like the full-output launches above.
+    # iteration counts) runs the lean single-wave BP kernel, whose zero-offset
+    # build drops the offset clamp; both builds must agree with the oracle
+    # exactly like the full-output launches above.
     import torch
 
-    print(f"[case] hard-output device decode, OSD order {osd_order}", flush=True)
+    print(f"[case] hard-output device decode, OSD order {osd_order}, offset {offset}", flush=True)
     _, _, llr = _frames(ccsds, 4000, 2.0, 515 + osd_order)
     dev = torch.device("cuda:0")
     d_llr = torch.from_numpy(llr).to(dev)
@@ -60,7 +62,7 @@ def test_hard_output_launch_matches_oracle(ccsds, osd_order):
     d_ok = torch.zeros(cw, dtype=torch.uint8, device=dev)
     d_used = torch.zeros(cw, dtype=torch.uint8, device=dev)
     d_it = torch.zeros(cw, dtype=torch.int32, device=dev)
-    with sa.Decoder.stream(sa.stream_config(max_iter=20, alpha=0.75)) as dec:
+    with sa.Decoder.stream(sa.stream_config(max_iter=20, alpha=0.75, offset=offset)) as dec:
         dec.set_osd(osd_order, 64, 4096)
         dec.decode_device(d_llr.data_ptr(), cw, d_info=d_info.data_ptr(), d_ok=d_ok.data_ptr(),
                           d_used=d_used.data_ptr(), d_iters=d_it.data_ptr(),
@@ -69,12 +71,13 @@ def test_hard_output_launch_matches_oracle(ccsds, osd_order):
     info, ok, used, its = (t.cpu().numpy() for t in (d_info, d_ok, d_used, d_it))
     if osd_order < 0:
         for i in range(0, cw, 7):
-            o = ccsds.nms(llr[i], max_iter=20, alpha=0.75)
+            o = ccsds.nms(llr[i], max_iter=20, alpha=0.75, offset=offset)
             assert ok[i] == o["ok"], i
             assert its[i] == o["iterations"], i
             assert np.array_equal(info[i], o["hard"][ccsds.info_cols]), i
     else:
-        o = ccsds.bposd(llr, max_iter=20, alpha=0.75, order=1, max_flip=64, max_tests=4096, threads=8)
+        o = ccsds.bposd(llr, max_iter=20, alpha=0.75, offset=offset, order=1, max_flip=64, max_tests=4096,
+                        threads=8)
         assert o["used_osd"].sum() > 10
         assert np.array_equal(used, o["used_osd"])
         assert np.array_equal(ok, o["bp_ok"])
