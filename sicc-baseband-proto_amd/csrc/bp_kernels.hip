@@ -615,11 +615,12 @@ constexpr int team_waves_per_eu()
     return MPL == 1 && ((TEAM == 16 && MAXDC <= 14) || (TEAM == 8 && MAXDC <= 12)) ? 8 : 1;
 }
 
-template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM, bool OFF>
 __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) __attribute__((amdgpu_waves_per_eu(
     team_waves_per_eu<MPL, MAXDC, TEAM>()))) void bp_flood_kernel(BpCodeDev code, BpParams p, BpIo io, int e_pad)
 {
-    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(code, p, io, e_pad, grid_team<64 * TEAM>(), grid_teams<64 * TEAM>());
+    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM, OFF>(code, p, io, e_pad, grid_team<64 * TEAM>(),
+                                                    grid_teams<64 * TEAM>());
 }
 
 // Several batches, of codes that share one instance of the decode loop, in one
@@ -640,7 +641,7 @@ struct BpSegmentTable {
     int count;
 };
 
-template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM>
+template <int MPL, int NPL, int MAXDC, bool DAMP, int TEAM, bool OFF>
 __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) __attribute__((amdgpu_waves_per_eu(
     team_waves_per_eu<MPL, MAXDC, TEAM>()))) void bp_flood_multi_kernel(BpSegmentTable t)
 {
@@ -649,7 +650,7 @@ __global__ __launch_bounds__(TEAM == 1 ? 256 : 64 * TEAM) __attribute__((amdgpu_
     while (si + 1 < t.count && static_cast<int>(blockIdx.x) >= t.seg[si + 1].block0) ++si;
     const BpSegment& g = t.seg[si];
     const int gteam = static_cast<int>((blockIdx.x - g.block0) * (blockDim.x / T) + threadIdx.x / T);
-    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM>(g.code, g.p, g.io, g.e_pad, gteam, g.teams);
+    bp_flood_body<MPL, NPL, MAXDC, DAMP, TEAM, OFF>(g.code, g.p, g.io, g.e_pad, gteam, g.teams);
 }
 
 // The single-wave shape of codes up to 128 variables and row degree 8 without
@@ -718,7 +719,12 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
     while (tpb > 1 && team_bytes * tpb > 64 * 1024) --tpb;
     if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
     const size_t lds = team_bytes * tpb;
-    auto kern = damp ? bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM>;
+    // damped launches take the general build (offset applied as given); undamped
+    // ones a build with or without the offset clamp
+    const bool off = p.offset != 0.0f;
+    auto kern = damp  ? bp_flood_kernel<MPL, NPL, MAXDC, true, TEAM, true>
+                : off ? bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM, true>
+                      : bp_flood_kernel<MPL, NPL, MAXDC, false, TEAM, false>;
     bool lean = false;
     if constexpr (MPL == 1 && NPL == 2 && MAXDC == 8 && TEAM == 1) {
         if (!damp) {
@@ -726,7 +732,6 @@ void launch_variant(const BpCodeDev& code, const BpParams& p, const BpIo& io, in
             kern = bp_flood_kernel_w8;
         }
     }
-    const bool off = p.offset != 0.0f;
     auto lean_kern = io.work_ctr ? (off ? bp_flood_kernel_lean<true, true> : bp_flood_kernel_lean<true, false>)
                                  : (off ? bp_flood_kernel_lean<false, true> : bp_flood_kernel_lean<false, false>);
     const void* kfn = lean ? reinterpret_cast<const void*>(lean_kern) : reinterpret_cast<const void*>(kern);
@@ -770,7 +775,10 @@ void launch_multi_variant(const BpBatch* b, int count, hipStream_t s)
     while (tpb > 1 && team_bytes * tpb > 64 * 1024) --tpb;
     if (team_bytes * tpb > 64 * 1024) throw std::invalid_argument("code too large for the LDS-resident BP kernel");
     const size_t lds = team_bytes * tpb;
-    auto kern = damp ? bp_flood_multi_kernel<MPL, NPL, MAXDC, true, TEAM> : bp_flood_multi_kernel<MPL, NPL, MAXDC, false, TEAM>;
+    const bool off = b[0].p.offset != 0.0f;
+    auto kern = damp  ? bp_flood_multi_kernel<MPL, NPL, MAXDC, true, TEAM, true>
+                : off ? bp_flood_multi_kernel<MPL, NPL, MAXDC, false, TEAM, true>
+                      : bp_flood_multi_kernel<MPL, NPL, MAXDC, false, TEAM, false>;
     const long resident =
         static_cast<long>(resident_blocks_per_cu(reinterpret_cast<const void*>(kern), T * tpb, lds)) * device_cu_count();
     long need[kBpSegmentsPerLaunch];
@@ -939,7 +947,9 @@ int bp_flood_instance(const BpCodeDev& c, const BpParams& p, const BpTune& tune)
     per_thread(c, team, &mpl, &npl);
     const bool small = mpl <= 1 && npl <= 2;
     const int tpb = team == 1 ? (tune.teams_per_block >= 4 ? 4 : (tune.teams_per_block >= 2 ? 2 : 1)) : 1;
-    return team << 12 | tpb << 8 | dc_class(c.max_dc) << 3 | (small ? 4 : 0) | (p.damping != 0.0f ? 1 : 0);
+    const bool damp = p.damping != 0.0f;
+    const bool off = !damp && p.offset != 0.0f;  // undamped launches have builds with and without the offset
+    return team << 12 | tpb << 8 | dc_class(c.max_dc) << 3 | (small ? 4 : 0) | (off ? 2 : 0) | (damp ? 1 : 0);
 }
 
 void launch_bp_flood_multi(const BpBatch* batches, int count, hipStream_t s)

@@ -146,8 +146,9 @@ struct BpBatch {
     BpTune tune;
     int max_waves = 0;
 };
-// Key of the kernel instance (launch shape, row-degree class, damping) a batch
-// runs on; batches with equal keys can share a launch.  -1: no instance fits.
+// Key of the kernel instance (launch shape, row-degree class, damping, and for
+// undamped batches whether the offset is nonzero) a batch runs on; batches with
+// equal keys can share a launch.  -1: no instance fits.
 int bp_flood_instance(const BpCodeDev& code, const BpParams& p, const BpTune& tune);
 // Decodes batches of one instance key, up to eight per launch, each with its
 // own code, parameters and outputs; results equal launch_bp_flood's per batch.
