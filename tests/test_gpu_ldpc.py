@@ -84,6 +84,43 @@ def test_hard_output_launch_matches_oracle(ccsds, osd_order, offset):
         assert np.array_equal(info, o["info"])
 
 
+@pytest.mark.parametrize("steal", [0, 4])
+@pytest.mark.parametrize("seed", [5, 8])
+def test_regular_weight3_hard_output_launch(seed, steal):
+    # A regular column-weight-3, rate-1/2 QC code of 128 variables whose rows
+    # hold at most 8 edges, with hard outputs only, runs the single-wave
+    # hard-output build (static striding and work stealing) on a second code
+    # shape; it must agree with the oracle exactly.
+    import torch
+
+    print(f"[case] regular weight-3 QC n=128 seed {seed}, steal chunk {steal}", flush=True)
+    n, mb = 128, 8
+    m, rp, ci = sa.qc_code(n, mb, 3, seed=seed)
+    assert np.all(np.bincount(ci, minlength=n) == 3) and np.diff(rp).max() <= 8
+    code = oracle.Code(n, m, rp, ci, front_info=True)
+    rng = np.random.default_rng(77 + seed)
+    cw = 3000
+    info = rng.integers(0, 2, (cw, code.k), dtype=np.uint8)
+    llr = oracle.awgn_llr(code.encode(info), 2.5, code.k / n, rng)
+    dev = torch.device("cuda:0")
+    d_llr = torch.from_numpy(llr).to(dev)
+    d_info = torch.zeros((cw, code.k), dtype=torch.uint8, device=dev)
+    d_ok = torch.zeros(cw, dtype=torch.uint8, device=dev)
+    d_it = torch.zeros(cw, dtype=torch.int32, device=dev)
+    with sa.Decoder.bp_csr(n, m, rp, ci, max_iter=30, alpha=0.75, schedule=0) as dec:
+        dec.set_tune(1, 4, 0, steal)
+        dec.decode_device(d_llr.data_ptr(), cw, d_info=d_info.data_ptr(), d_ok=d_ok.data_ptr(),
+                          d_iters=d_it.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+    got_info, ok, its = (t.cpu().numpy() for t in (d_info, d_ok, d_it))
+    assert 0 < ok.sum() < cw
+    for i in range(cw):
+        o = code.nms(llr[i], max_iter=30, alpha=0.75)
+        assert ok[i] == o["ok"], i
+        assert its[i] == o["iterations"], i
+        assert np.array_equal(got_info[i], o["hard"][code.info_cols]), i
+
+
 def test_full_headline_batch_properties(ccsds):
     # The headline workload at full size (1 048 576 codewords, Eb/N0 4.0 dB, BP 50
     # iterations + order-1 OSD): every output is a codeword (H c = 0 for BP
