@@ -14,8 +14,15 @@ struct Cf {
     float x, y;
 };
 
-static __device__ __forceinline__ Cd cmul(Cd a, Cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-static __device__ __forceinline__ Cf cmul(Cf a, Cf b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// Complex products with one fused multiply-add per component (2 multiplies and
+// 2 FMAs instead of 4 multiplies and 2 adds; one rounding fewer per component).
+// The spectra are compared with the reference's within a relative tolerance,
+// not bit for bit.
+static __device__ __forceinline__ Cd cmul(Cd a, Cd b) { return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)}; }
+static __device__ __forceinline__ Cf cmul(Cf a, Cf b)
+{
+    return {fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x)};
+}
 
 // Radix-4 DFT of (a0, a1, a2, a3) in place: the forward 4-point transform
 // (W4 = -i).

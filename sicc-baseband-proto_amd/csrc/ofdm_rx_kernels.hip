@@ -385,8 +385,8 @@ __device__ void symbol_fft(const OfdmDemodArgs& a, const float2* x, double cfo, 
         const float re = v.x * c - v.y * s;
         const float im = v.x * s + v.y * c;
         buf[k] = {static_cast<double>(re), static_cast<double>(im)};
-        const double cn = c0 * cd - s0 * sd;
-        s0 = s0 * cd + c0 * sd;
+        const double cn = fma(c0, cd, -(s0 * sd));
+        s0 = fma(s0, cd, c0 * sd);
         c0 = cn;
     }
     __syncthreads();
