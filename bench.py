@@ -324,10 +324,10 @@ def serve_sweep(a, sa, np, rng, decoders, members, counts, ptrs, ks, slots, max_
             return round((s1.jobs_completed - s0.jobs_completed) / dl, 2) if dl else None
 
         # warm-up: the first drive fills the server's recycled result-buffer pool
-        # (fresh heap pages before that); steady-state serving reuses it (three
-        # drives: in a process that has run the other legs first, the copies into
-        # the recycled buffers still ran slower for two more drives)
-        for _ in range(3):
+        # (fresh heap pages before that); steady-state serving reuses it (eight
+        # drives of about 5 ms each: in a process that has run the other legs
+        # first, drives ran 30 % slower for up to six drives)
+        for _ in range(8):
             srv.drive(members, counts, ptrs, np.zeros(n_jobs))
         # peak: median of five all-at-once drives (one drive lasts tens of ms, so a
         # single stall would set every offered load below)
