@@ -217,15 +217,28 @@ __device__ __forceinline__ void elim_step(unsigned long long (&col)[CPL][KW], un
             if (lane == (p & 63)) lrow[q] = t;
         }
     }
+    // bit p of a column word as an all-ones / zero mask: the 32-bit half holding it
+    // (p is the same in every lane), then a sign-extending one-bit field extract;
+    // the masked XOR is then one three-input op per 32-bit half
+    const bool phi = (p & 32) != 0;
+    const int pb = p & 31;
 #pragma unroll
     for (int s = 0; s < CPL; ++s) {
         unsigned long long pw = col[s][0];
 #pragma unroll
         for (int q = 1; q < KW; ++q)
             if (q == pq) pw = col[s][q];
-        const bool hit = (pw & pbit) != 0ull;
+        const unsigned half = phi ? static_cast<unsigned>(pw >> 32) : static_cast<unsigned>(pw);
+        const unsigned m = static_cast<unsigned>(__builtin_amdgcn_sbfe(static_cast<int>(half), pb, 1));
 #pragma unroll
-        for (int q = 0; q < KW; ++q) col[s][q] ^= hit ? clr[q] : 0ull;
+        for (int q = 0; q < KW; ++q) {
+            // w ^ (c & m) per half: v_bitop3 truth table 0x78
+            const unsigned lo = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(col[s][q]),
+                                                            static_cast<unsigned>(clr[q]), m, 0x78);
+            const unsigned hi = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(col[s][q] >> 32),
+                                                            static_cast<unsigned>(clr[q] >> 32), m, 0x78);
+            col[s][q] = (static_cast<unsigned long long>(hi) << 32) | lo;
+        }
     }
 }
 
